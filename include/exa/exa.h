@@ -1,0 +1,219 @@
+/*
+ * exa.h -- C ABI of the MI355X-native exaStitch volume path tracer
+ * (libexastitch.so).  Plain C types only; every entry point returns an int
+ * status (EXA_OK == 0, negative on error) and never throws.  The message of
+ * the last failure on the calling thread is exa_get_last_error().
+ *
+ * The boundary mirrors the reference's three host-visible layers:
+ *   - the model      exa::ExaStitchModel   (model/ExaStitchModel.h:24-57,
+ *                                           model/Model.h:29-65)
+ *   - the sampler    exa::ExaStitchSampler (sampler/Sampler.h:28-69,
+ *                                           sampler/ExaStitchSampler.h:37-141)
+ *   - the renderer   exa::OWLRenderer      (OWLRenderer.h:29-175)
+ * plus the offline dual-mesh tools (tools/amrMakeDualMesh) and synthetic
+ * scene generators.  Device pointers are hipMalloc'd memory of the current
+ * device; `stream` is a hipStream_t passed as void* (NULL = default stream).
+ * Objects are not thread-safe (the reference's OWLRenderer is used from one
+ * thread, OWLRenderer.cpp).
+ */
+#ifndef EXA_EXA_H
+#define EXA_EXA_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EXA_OK 0
+#define EXA_ERR_INVALID (-1) /* bad argument or invalid model data */
+#define EXA_ERR_HIP (-2)     /* HIP runtime failure */
+#define EXA_ERR_STATE (-3)   /* call order (e.g. locate before build) */
+#define EXA_ERR_ALLOC (-4)   /* host or device allocation failed */
+
+/* Gridlet.h:22-37 (32 bytes) */
+typedef struct {
+  int32_t lower[3];
+  int32_t level;
+  int32_t dims[3];
+  uint32_t begin;
+} exa_gridlet;
+
+/* sampler/Sampler.h:72-76 (12 bytes) */
+typedef struct {
+  int32_t primID;
+  int32_t cellID;
+  float value;
+} exa_sample;
+
+/* tools/amrMakeDualMesh/makeDual.cpp:75-84 LogicalCell (16 bytes) */
+typedef struct {
+  int32_t pos[3];
+  int32_t level;
+} exa_amr_cell;
+
+const char *exa_get_last_error(void);
+int exa_version(void);
+
+/* ---- synthetic AMR scenes (no reference counterpart: generators of the
+        benchmark inputs; the reference reads .cells/.scalars files) ---- */
+typedef struct exa_amr exa_amr;
+int exa_amr_create_gaussians(int baseRes, int coarseLevel, int fineLo, int fineHi, int numGaussians,
+                             uint64_t seed, exa_amr **out);
+int exa_amr_create_noise(int baseRes, int baseLevel, float thresh1, float thresh2, int octaves,
+                         float contrast, uint64_t seed, exa_amr **out);
+int exa_amr_create_from_cells(const exa_amr_cell *cells, const float *scalars, int64_t n, exa_amr **out);
+int64_t exa_amr_num_cells(const exa_amr *a);
+int exa_amr_get(const exa_amr *a, exa_amr_cell *cells, float *scalars);
+int exa_amr_get_params(const exa_amr *a, float *out, int maxCount);
+void exa_amr_destroy(exa_amr *a);
+
+/* ---- dual mesh: replaces amrMakeDualMesh + amrMakeGrids
+        (tools/amrMakeDualMesh/makeDual.cpp:596-1000, makeGrids.cpp:46-343) ---- */
+typedef struct exa_dual_mesh exa_dual_mesh;
+int exa_dual_mesh_build(const exa_amr *a, exa_dual_mesh **out);
+/* counts[11]: tets, pyramids, wedges, hexes, elements, vertices, gridlets,
+   gridlet scalars, perfect cubes, conflicts, degenerate cells dropped */
+int exa_dual_mesh_counts(const exa_dual_mesh *d, int64_t counts[11]);
+int exa_dual_mesh_get(const exa_dual_mesh *d, int32_t *indices, float *vertices, exa_gridlet *gridlets,
+                      float *gridletScalars);
+void exa_dual_mesh_destroy(exa_dual_mesh *d);
+
+/* ---- model: exa::ExaStitchModel (ExaStitchModel.cpp:24-257) ---- */
+typedef struct exa_model exa_model;
+typedef struct {
+  const float *vertices; /* float4: xyz + scalar */
+  int64_t numVertices;
+  const int32_t *indices; /* int[8] per element, -1 padded; tets, pyrs, wedges, hexes */
+  int64_t numElements;
+  const exa_gridlet *gridlets;
+  int64_t numGridlets;
+  const float *gridletScalars; /* NaN = empty */
+  int64_t numGridletScalars;
+} exa_stitch_model_desc;
+int exa_model_create(const exa_stitch_model_desc *desc, exa_model **out); /* copies */
+int exa_model_create_from_dual_mesh(const exa_dual_mesh *d, exa_model **out);
+/* cellBounds (lo xyz, hi xyz) and valueRange (Model.h:42-43) */
+int exa_model_get_bounds(const exa_model *m, float cellBounds[6], float valueRange[2]);
+int exa_model_get_counts(const exa_model *m, int64_t counts[4]); /* vertices, elements, gridlets, scalars */
+int exa_model_set_num_grid_cells(exa_model *m, const int32_t numMCs[3]);                     /* Model.cpp:25-33 */
+int exa_model_set_voxel_space_transform(exa_model *m, const float from[6], const float to[6]); /* Model.cpp:35-48 */
+void exa_model_destroy(exa_model *m);
+
+/* ---- sampler: exa::ExaStitchSampler ---- */
+typedef struct exa_sampler exa_sampler;
+int exa_sampler_create(exa_sampler **out);
+/* Sampler::build (ExaStitchSampler.cpp:23-340): uploads the model, builds the
+   point-location BVHs and the majorant grid over the model's cellBounds with
+   the model's numMCs (Grid.cu:457-594).  Fails with EXA_ERR_STATE when numMCs
+   is unset or zero (ExaStitchSampler.cpp:326-327). */
+int exa_sampler_build(exa_sampler *s, const exa_model *m, void *stream);
+/* Sampler::computeMaxOpacities (ExaStitchSampler.cu:279-376): rgba is host
+   float4[numColors]; xfRange = [lo, hi]. */
+int exa_sampler_compute_max_opacities(exa_sampler *s, const float *rgba, int numColors, float lo, float hi,
+                                      void *stream);
+/* device sample() (ExaStitchSampler.h:128-140) over n device points (float3). */
+int exa_sampler_locate(const exa_sampler *s, const float *d_pos, int64_t n, exa_sample *d_out, void *stream);
+/* same, accumulating per-result counts into host counters[6]: gridlet, tet,
+   pyramid, wedge, hex, miss (synchronises the stream). */
+int exa_sampler_locate_counted(const exa_sampler *s, const float *d_pos, int64_t n, exa_sample *d_out,
+                               uint64_t counters[6], void *stream);
+/* device testSample() (ExaStitchSampler.h:109-126) with per-point gridlet ids. */
+int exa_sampler_test_sample(const exa_sampler *s, const float *d_pos, const int32_t *d_primIDs, int64_t n,
+                            exa_sample *d_out, void *stream);
+int exa_sampler_get_grid_info(const exa_sampler *s, int32_t dims[3], float bounds[6]);
+int exa_sampler_get_value_ranges(const exa_sampler *s, float *out);          /* host, 2 per macrocell */
+int exa_sampler_get_max_opacities(const exa_sampler *s, float *out);         /* host, 1 per macrocell */
+int exa_sampler_get_gridlet_value_ranges(const exa_sampler *s, float *out);  /* host, 2 per gridlet */
+int exa_sampler_get_prim_max_opacities(const exa_sampler *s, float *gridletOut, float *elementOut);
+/* info[8]: gridlet BVH nodes, depth, refs; element BVH nodes, depth, refs;
+   device bytes of the model; device bytes of the BVHs */
+int exa_sampler_get_info(const exa_sampler *s, int64_t info[8]);
+void exa_sampler_destroy(exa_sampler *s);
+
+/* ---- renderer: exa::OWLRenderer ---- */
+typedef struct exa_renderer exa_renderer;
+/* OWLRenderer(...) + initGPU (OWLRenderer.cpp:123-181, 309-582); the model is
+   copied; numMCs = macrocell grid dims; remapFrom/remapTo as lo xyz, hi xyz
+   (NULL = unit boxes). */
+int exa_renderer_create(const exa_model *m, const int32_t numMCs[3], const float remapFrom[6],
+                        const float remapTo[6], exa_renderer **out);
+int exa_renderer_set_camera(exa_renderer *r, const float org[3], const float dir00[3], const float du[3],
+                            const float dv[3]);
+int exa_renderer_resize(exa_renderer *r, int width, int height);
+/* render(uint32_t *fbPointer) (OWLRenderer.cpp:610-622): one launch of
+   width*height pixels; d_fb may be NULL (accumulation only).  Asynchronous
+   on `stream`; accumID advances by one. */
+int exa_renderer_render(exa_renderer *r, uint32_t *d_fb, void *stream);
+/* render() with the instrumented kernel; counters[10] receive: gridlet, tet,
+   pyramid, wedge, hex hits, misses, cached gridlet hits, DDA cells, pixels,
+   TF fetches (synchronises the stream). */
+int exa_renderer_render_counted(exa_renderer *r, uint32_t *d_fb, uint64_t counters[10], void *stream);
+int exa_renderer_get_accum(const exa_renderer *r, float *hostOut); /* float4 per pixel */
+int exa_renderer_set_type(exa_renderer *r, int type); /* 0 path tracer, 1 direct lighting */
+int exa_renderer_set_color_map(exa_renderer *r, const float *rgba, int numColors);
+int exa_renderer_set_range(exa_renderer *r, float lo, float hi);
+int exa_renderer_set_rel_domain(exa_renderer *r, float lo, float hi);
+int exa_renderer_set_opacity_scale(exa_renderer *r, float scale); /* stores 1.1^(scale-100) */
+int exa_renderer_set_clip_plane(exa_renderer *r, int id, int enabled, const float N[3], float d);
+int exa_renderer_set_shade_mode(exa_renderer *r, int mode);
+int exa_renderer_set_light_source(exa_renderer *r, int id, const float pos[3], float intensity, int on);
+/* build-side addition: a light at infinity in direction dir (towards the light) */
+int exa_renderer_set_directional_light(exa_renderer *r, int id, const float dir[3], float intensity, int on);
+int exa_renderer_set_light_space_transform(exa_renderer *r, const float xfm[12]);
+int exa_renderer_set_sub_image(exa_renderer *r, const float box[4], int active);
+int exa_renderer_set_sub_image_selection(exa_renderer *r, const float box[4], int active);
+int exa_renderer_reset_accum(exa_renderer *r);
+int exa_renderer_set_spp(exa_renderer *r, int spp);
+/* deviceCode.cu:1050 hard-codes 1024 bounces; the default here is 1024 too */
+int exa_renderer_set_max_bounces(exa_renderer *r, int maxBounces);
+/* deviceCode.cu:1334 inverts the frame-buffer pixels of the centre row and
+   column (DEBUGGING == 1 in common.h); on by default, as in the reference */
+int exa_renderer_set_debug_crosshair(exa_renderer *r, int on);
+int exa_renderer_get_bounds(const exa_renderer *r, float modelBounds[6]);
+int exa_renderer_get_value_range(const exa_renderer *r, float valueRange[2]);
+int exa_renderer_get_accum_id(const exa_renderer *r);
+const exa_sampler *exa_renderer_get_sampler(const exa_renderer *r);
+
+/* Snapshot of the launch parameters of the next render() call (for checking
+   against the CPU oracle). */
+typedef struct {
+  int32_t W, H, spp, accumID;
+  int32_t integrator, shadeMode, maxBounces, debugCrosshair;
+  float camOrg[3], camDir00[3], camDu[3], camDv[3];
+  float worldBounds[6];
+  float voxelXfm[12], voxelXfmInv[12], lightXfm[12];
+  int32_t gridDims[3];
+  float gridBounds[6];
+  float tfDomain[2];
+  float opacityScale;
+  int32_t tfN;
+  int32_t clipEnabled;
+  float clipN[3];
+  float clipD;
+  float lightPos[4][3];
+  float lightIntensity[4];
+  int32_t lightOn[4];
+  int32_t lightType[4];
+} exa_render_state;
+int exa_renderer_get_state(const exa_renderer *r, exa_render_state *out);
+void exa_renderer_destroy(exa_renderer *r);
+
+/* ---- host helpers ---- */
+/* Camera from eye/point-of-interest/up/fovy for a W x H image
+   (viewer.cpp:79-107 SimpleCamera, 550-560 cameraChanged). */
+int exa_camera_look_at(const float from[3], const float at[3], const float up[3], float fovyDeg, int W, int H,
+                       float org[3], float dir00[3], float du[3], float dv[3]);
+/* .xf transfer-function file (headless.cpp:258-283).  Pass rgba == NULL to
+   query numColors. */
+int exa_xf_load(const char *path, float *opacityScale, float absDomain[2], float relDomain[2], int *numColors,
+                float *rgba, int maxColors);
+/* Piecewise-linear RGBA transfer function: points are (x, r, g, b, a) with x
+   ascending in [0,1]; rasterised to N texels at x = i/(N-1). */
+int exa_tf_rasterize(const float *points, int numPoints, int N, float *rgba);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,11 @@
+"""MI355X-native exaStitch volume path tracer (hot path of owlExaStitcher).
+
+All computation lives in the in-tree HIP library ``libexastitch.so``
+(``owlexastitcher_amd/csrc``, C ABI in ``include/exa/exa.h``); this package
+is its Python host mirror.  Importing the host classes loads the library and
+raises ImportError if it has not been built.
+"""
+from ._capi import LIB_PATH, ExaError, lib  # noqa: F401
+from .host import (DIRECT_LIGHTING, PATH_TRACER, AmrScene, DualMesh,  # noqa: F401
+                   ExaStitchModel, ExaStitchSampler, OWLRenderer, camera_look_at,
+                   tf_rasterize, xf_load)

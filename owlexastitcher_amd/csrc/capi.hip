@@ -1,0 +1,929 @@
+// capi.hip -- extern "C" boundary of libexastitch.so (declared in
+// include/exa/exa.h) and the host objects behind it: ExaStitchModel,
+// ExaStitchSampler (+ its majorant Grid) and OWLRenderer.
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+#include <exception>
+#include <fstream>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+#include "../../include/exa/exa.h"
+#include "exa_kernels.h"
+
+namespace exa {
+
+static thread_local std::string g_lastError;
+
+struct Status : std::runtime_error {
+  int code;
+  Status(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIP_CHECK(x)                                                                                     \
+  do {                                                                                                   \
+    hipError_t e_ = (x);                                                                                 \
+    if (e_ != hipSuccess)                                                                                \
+      throw Status(EXA_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_));                         \
+  } while (0)
+
+template <typename F>
+static int guarded(F &&f) {
+  try {
+    f();
+    return EXA_OK;
+  } catch (const Status &s) {
+    g_lastError = s.what();
+    return s.code;
+  } catch (const std::bad_alloc &) {
+    g_lastError = "host allocation failed";
+    return EXA_ERR_ALLOC;
+  } catch (const std::exception &e) {
+    g_lastError = e.what();
+    return EXA_ERR_INVALID;
+  } catch (...) {
+    g_lastError = "unknown error";
+    return EXA_ERR_INVALID;
+  }
+}
+
+static void require(bool c, const char *msg) {
+  if (!c) throw Status(EXA_ERR_INVALID, msg);
+}
+
+template <typename T>
+struct DeviceBuffer {
+  T *ptr = nullptr;
+  size_t count = 0;
+  DeviceBuffer() = default;
+  DeviceBuffer(const DeviceBuffer &) = delete;
+  DeviceBuffer &operator=(const DeviceBuffer &) = delete;
+  ~DeviceBuffer() { release(); }
+  void release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    count = 0;
+  }
+  void alloc(size_t n) {
+    if (n == count && ptr) return;
+    release();
+    if (n == 0) return;
+    hipError_t e = hipMalloc((void **)&ptr, n * sizeof(T));
+    if (e != hipSuccess) {
+      ptr = nullptr;
+      throw Status(EXA_ERR_ALLOC, std::string("hipMalloc of ") + std::to_string(n * sizeof(T)) +
+                                      " bytes failed: " + hipGetErrorString(e));
+    }
+    count = n;
+  }
+  void upload(const T *h, size_t n, hipStream_t s) {
+    alloc(n);
+    if (n) HIP_CHECK(hipMemcpyAsync(ptr, h, n * sizeof(T), hipMemcpyHostToDevice, s));
+  }
+  size_t bytes() const { return count * sizeof(T); }
+};
+
+// ---------------------------------------------------------------------------
+// affine helpers (owl affine3f: columns vx, vy, vz and translation p)
+// ---------------------------------------------------------------------------
+static void affineIdentity(float a[12]) {
+  static const float I[12] = {1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0};
+  memcpy(a, I, sizeof(I));
+}
+// general inverse, evaluated in double and rounded
+static void affineInverse(const float a[12], float out[12]) {
+  const double m[3][3] = {{a[0], a[3], a[6]}, {a[1], a[4], a[7]}, {a[2], a[5], a[8]}};  // row-major
+  const double det = m[0][0] * (m[1][1] * m[2][2] - m[1][2] * m[2][1]) -
+                     m[0][1] * (m[1][0] * m[2][2] - m[1][2] * m[2][0]) +
+                     m[0][2] * (m[1][0] * m[2][1] - m[1][1] * m[2][0]);
+  if (det == 0.0) throw Status(EXA_ERR_INVALID, "singular voxel-space transform");
+  double inv[3][3];
+  inv[0][0] = (m[1][1] * m[2][2] - m[1][2] * m[2][1]) / det;
+  inv[0][1] = (m[0][2] * m[2][1] - m[0][1] * m[2][2]) / det;
+  inv[0][2] = (m[0][1] * m[1][2] - m[0][2] * m[1][1]) / det;
+  inv[1][0] = (m[1][2] * m[2][0] - m[1][0] * m[2][2]) / det;
+  inv[1][1] = (m[0][0] * m[2][2] - m[0][2] * m[2][0]) / det;
+  inv[1][2] = (m[0][2] * m[1][0] - m[0][0] * m[1][2]) / det;
+  inv[2][0] = (m[1][0] * m[2][1] - m[1][1] * m[2][0]) / det;
+  inv[2][1] = (m[0][1] * m[2][0] - m[0][0] * m[2][1]) / det;
+  inv[2][2] = (m[0][0] * m[1][1] - m[0][1] * m[1][0]) / det;
+  double t[3];
+  for (int r = 0; r < 3; ++r) t[r] = -(inv[r][0] * a[9] + inv[r][1] * a[10] + inv[r][2] * a[11]);
+  for (int c = 0; c < 3; ++c)
+    for (int r = 0; r < 3; ++r) out[3 * c + r] = (float)inv[r][c];
+  for (int r = 0; r < 3; ++r) out[9 + r] = (float)t[r];
+}
+static void xfmPointHost(const float a[12], const float p[3], float out[3]) {
+  for (int k = 0; k < 3; ++k) out[k] = a[k] * p[0] + (a[3 + k] * p[1] + (a[6 + k] * p[2] + a[9 + k]));
+}
+
+template <typename T>
+static void download(const DeviceBuffer<T> &b, void *out) {
+  if (!out || !b.count) return;
+  HIP_CHECK(hipDeviceSynchronize());
+  HIP_CHECK(hipMemcpy(out, b.ptr, b.bytes(), hipMemcpyDeviceToHost));
+}
+
+}  // namespace exa
+
+using namespace exa;
+
+// ---------------------------------------------------------------------------
+// opaque objects
+// ---------------------------------------------------------------------------
+struct exa_amr {
+  AmrScene scene;
+};
+struct exa_dual_mesh {
+  DualMesh mesh;
+};
+struct exa_model {
+  HostModel m;
+};
+
+struct exa_sampler {
+  // model copy on the device
+  DeviceBuffer<float4> vertices;
+  DeviceBuffer<int32_t> indices;
+  DeviceBuffer<Gridlet> gridlets;
+  DeviceBuffer<float> gridletScalars;
+  int64_t numElements = 0, numGridlets = 0;
+  // point-location BVHs
+  DeviceBuffer<BvhNode> gNodes, eNodes;
+  DeviceBuffer<int32_t> gRefs, eRefs;
+  int gDepth = 0, eDepth = 0;
+  // gridlet value ranges + per-primitive max opacities (culling)
+  DeviceBuffer<float2> gridletRanges;
+  DeviceBuffer<float> gridletMaxOp, elemMaxOp;
+  bool haveMaxOp = false;
+  // majorant grid (Grid.h:33-86)
+  int32_t dims[3] = {0, 0, 0};
+  float bounds[6] = {0, 0, 0, 0, 0, 0};
+  DeviceBuffer<float2> valueRanges;
+  DeviceBuffer<float> maxOpacities;
+  DeviceBuffer<float4> colorMap;
+  DeviceBuffer<unsigned long long> counters;
+  bool built = false;
+
+  SamplerView view() const {
+    SamplerView v;
+    v.gridlets = gridlets.ptr;
+    v.gridletScalars = gridletScalars.ptr;
+    v.gridletMaxOp = haveMaxOp ? gridletMaxOp.ptr : nullptr;
+    v.gNodes = gNodes.ptr;
+    v.gRefs = gRefs.ptr;
+    v.numGridlets = (int32_t)numGridlets;
+    v.vertices = vertices.ptr;
+    v.indices = indices.ptr;
+    v.elemMaxOp = haveMaxOp ? elemMaxOp.ptr : nullptr;
+    v.eNodes = eNodes.ptr;
+    v.eRefs = eRefs.ptr;
+    v.numElements = (int32_t)numElements;
+    return v;
+  }
+  GridView gridView() const {
+    GridView g;
+    for (int k = 0; k < 3; ++k) g.dims[k] = dims[k];
+    for (int k = 0; k < 6; ++k) g.bounds[k] = bounds[k];
+    g.majorants = maxOpacities.ptr;
+    return g;
+  }
+
+  // ExaStitchSampler::build (ExaStitchSampler.cpp:23-340) + Grid::build
+  // (Grid.cu:457-594)
+  void build(const HostModel &m, hipStream_t s) {
+    built = false;
+    haveMaxOp = false;
+    if (m.gridlets.empty() && m.indices.empty())
+      throw Status(EXA_ERR_INVALID, "model has neither gridlets nor elements");
+    if (m.numMCs[0] <= 0 || m.numMCs[1] <= 0 || m.numMCs[2] <= 0)
+      throw Status(EXA_ERR_STATE, "number of macrocells not set (Model::setNumGridCells)");
+    if (m.numElements() >= (1ll << 31) || (int64_t)m.gridlets.size() >= (1ll << 31))
+      throw Status(EXA_ERR_INVALID, "more than 2^31 primitives");
+    numElements = m.numElements();
+    numGridlets = (int64_t)m.gridlets.size();
+    vertices.upload((const float4 *)m.vertices.data(), m.vertices.size() / 4, s);
+    indices.upload(m.indices.data(), m.indices.size(), s);
+    gridlets.upload(m.gridlets.data(), m.gridlets.size(), s);
+    gridletScalars.upload(m.gridletScalars.data(), m.gridletScalars.size(), s);
+    // BVHs (the OptiX BLASes over GridletGeomBounds / StitchGeomBounds)
+    std::vector<float> boxes;
+    Bvh bvh;
+    gridletBoxes(m.gridlets, boxes);
+    buildBvh(boxes.data(), numGridlets, 4, bvh);
+    gNodes.upload(bvh.nodes.data(), bvh.nodes.size(), s);
+    gRefs.upload(bvh.primRefs.data(), bvh.primRefs.size(), s);
+    gDepth = bvh.depth;
+    HIP_CHECK(hipStreamSynchronize(s));
+    elementBoxes(m.indices, m.vertices, boxes);
+    buildBvh(boxes.data(), numElements, 4, bvh);
+    eNodes.upload(bvh.nodes.data(), bvh.nodes.size(), s);
+    eRefs.upload(bvh.primRefs.data(), bvh.primRefs.size(), s);
+    eDepth = bvh.depth;
+    // computeGridletValueRanges (ExaStitchSampler.cu:229-277)
+    gridletRanges.alloc(numGridlets);
+    gridletMaxOp.alloc(numGridlets);
+    elemMaxOp.alloc(numElements);
+    launchGridletValueRanges(gridlets.ptr, gridletScalars.ptr, numGridlets, gridletRanges.ptr, s);
+    // majorant grid over the model's cellBounds (ExaStitchSampler.cpp:329)
+    for (int k = 0; k < 3; ++k) dims[k] = m.numMCs[k];
+    for (int k = 0; k < 6; ++k) bounds[k] = m.cellBounds[k];
+    const int64_t nmc = (int64_t)dims[0] * dims[1] * dims[2];
+    valueRanges.alloc(nmc);
+    maxOpacities.alloc(nmc);
+    HIP_CHECK(hipMemsetAsync(maxOpacities.ptr, 0, nmc * sizeof(float), s));
+    DeviceBuffer<uint2> scratch;
+    scratch.alloc(nmc);
+    launchGridBuild(view(), dims, bounds, valueRanges.ptr, scratch.ptr, s);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipStreamSynchronize(s));
+    counters.alloc(kNumCounters);
+    built = true;
+  }
+
+  // ExaStitchSampler::computeMaxOpacities (ExaStitchSampler.cu:279-376)
+  void computeMaxOpacities(const float *rgba, int n, float lo, float hi, hipStream_t s) {
+    if (!built) throw Status(EXA_ERR_STATE, "sampler not built");
+    require(rgba && n > 0, "empty color map");
+    colorMap.upload((const float4 *)rgba, n, s);
+    const int64_t nmc = (int64_t)dims[0] * dims[1] * dims[2];
+    launchMapMaxOpacity(valueRanges.ptr, nmc, colorMap.ptr, n, lo, hi, maxOpacities.ptr, s);
+    launchMapMaxOpacity(gridletRanges.ptr, numGridlets, colorMap.ptr, n, lo, hi, gridletMaxOp.ptr, s);
+    launchElementMaxOpacity(vertices.ptr, indices.ptr, numElements, colorMap.ptr, n, lo, hi, elemMaxOp.ptr, s);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipStreamSynchronize(s));
+    haveMaxOp = true;
+  }
+};
+
+struct exa_renderer {
+  HostModel model;
+  exa_sampler sampler;
+  RenderParams P;
+  DeviceBuffer<float4> accum;
+  DeviceBuffer<float4> tf;
+  std::vector<float> colorMap;
+  float absDomain[2] = {0.f, 1.f};
+  float relDomain[2] = {0.f, 100.f};
+  float modelBounds[6];
+  float valueRange[2];
+  int accumID = 0;
+  int spp = 1;
+
+  void updateMaxOpacities() {
+    if (colorMap.empty()) return;
+    // OWLRenderer.cpp:647-652
+    const float lo = absDomain[0] + (relDomain[0] / 100.f) * (absDomain[1] - absDomain[0]);
+    const float hi = absDomain[0] + (relDomain[1] / 100.f) * (absDomain[1] - absDomain[0]);
+    sampler.computeMaxOpacities(colorMap.data(), (int)colorMap.size() / 4, lo, hi, 0);
+  }
+};
+
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char *exa_get_last_error(void) { return g_lastError.c_str(); }
+int exa_version(void) { return 1; }
+
+// ---- AMR ------------------------------------------------------------------
+int exa_amr_create_gaussians(int baseRes, int coarseLevel, int fineLo, int fineHi, int numGaussians,
+                             uint64_t seed, exa_amr **out) {
+  return guarded([&] {
+    require(out, "out is NULL");
+    auto a = std::make_unique<exa_amr>();
+    amrGaussians(a->scene, baseRes, coarseLevel, fineLo, fineHi, numGaussians, seed);
+    *out = a.release();
+  });
+}
+int exa_amr_create_noise(int baseRes, int baseLevel, float t1, float t2, int octaves, float contrast,
+                         uint64_t seed, exa_amr **out) {
+  return guarded([&] {
+    require(out, "out is NULL");
+    auto a = std::make_unique<exa_amr>();
+    amrNoise(a->scene, baseRes, baseLevel, t1, t2, octaves, contrast, seed);
+    *out = a.release();
+  });
+}
+int exa_amr_create_from_cells(const exa_amr_cell *cells, const float *scalars, int64_t n, exa_amr **out) {
+  return guarded([&] {
+    require(out && cells && scalars && n > 0, "invalid cells");
+    auto a = std::make_unique<exa_amr>();
+    a->scene.cells.assign((const AmrCell *)cells, (const AmrCell *)cells + n);
+    a->scene.scalars.assign(scalars, scalars + n);
+    *out = a.release();
+  });
+}
+int64_t exa_amr_num_cells(const exa_amr *a) { return a ? (int64_t)a->scene.cells.size() : -1; }
+int exa_amr_get(const exa_amr *a, exa_amr_cell *cells, float *scalars) {
+  return guarded([&] {
+    require(a, "amr is NULL");
+    if (cells) memcpy(cells, a->scene.cells.data(), a->scene.cells.size() * sizeof(AmrCell));
+    if (scalars) memcpy(scalars, a->scene.scalars.data(), a->scene.scalars.size() * sizeof(float));
+  });
+}
+int exa_amr_get_params(const exa_amr *a, float *out, int maxCount) {
+  if (!a) return EXA_ERR_INVALID;
+  const int n = (int)a->scene.params.size();
+  if (out)
+    for (int i = 0; i < n && i < maxCount; ++i) out[i] = a->scene.params[i];
+  return n;
+}
+void exa_amr_destroy(exa_amr *a) { delete a; }
+
+// ---- dual mesh -------------------------------------------------------------
+int exa_dual_mesh_build(const exa_amr *a, exa_dual_mesh **out) {
+  return guarded([&] {
+    require(a && out, "invalid arguments");
+    auto d = std::make_unique<exa_dual_mesh>();
+    buildDualMesh(a->scene.cells.data(), a->scene.scalars.data(), (int64_t)a->scene.cells.size(), d->mesh);
+    *out = d.release();
+  });
+}
+int exa_dual_mesh_counts(const exa_dual_mesh *d, int64_t c[11]) {
+  return guarded([&] {
+    require(d && c, "invalid arguments");
+    const DualMesh &m = d->mesh;
+    for (int t = 0; t < 4; ++t) c[t] = m.counts[t];
+    c[4] = (int64_t)m.indices.size() / 8;
+    c[5] = (int64_t)m.vertices.size() / 4;
+    c[6] = (int64_t)m.gridlets.size();
+    c[7] = (int64_t)m.gridletScalars.size();
+    c[8] = m.numCubes;
+    c[9] = m.conflicts;
+    c[10] = m.degenerate;
+  });
+}
+int exa_dual_mesh_get(const exa_dual_mesh *d, int32_t *indices, float *vertices, exa_gridlet *gridlets,
+                      float *gridletScalars) {
+  return guarded([&] {
+    require(d, "dual mesh is NULL");
+    const DualMesh &m = d->mesh;
+    if (indices) memcpy(indices, m.indices.data(), m.indices.size() * sizeof(int32_t));
+    if (vertices) memcpy(vertices, m.vertices.data(), m.vertices.size() * sizeof(float));
+    if (gridlets) memcpy(gridlets, m.gridlets.data(), m.gridlets.size() * sizeof(Gridlet));
+    if (gridletScalars) memcpy(gridletScalars, m.gridletScalars.data(), m.gridletScalars.size() * sizeof(float));
+  });
+}
+void exa_dual_mesh_destroy(exa_dual_mesh *d) { delete d; }
+
+// ---- model -----------------------------------------------------------------
+int exa_model_create(const exa_stitch_model_desc *desc, exa_model **out) {
+  return guarded([&] {
+    require(desc && out, "invalid arguments");
+    require(desc->numVertices >= 0 && desc->numElements >= 0 && desc->numGridlets >= 0 &&
+                desc->numGridletScalars >= 0,
+            "negative counts");
+    require(!desc->numVertices || desc->vertices, "vertices missing");
+    require(!desc->numElements || desc->indices, "indices missing");
+    require(!desc->numGridlets || desc->gridlets, "gridlets missing");
+    require(!desc->numGridletScalars || desc->gridletScalars, "gridlet scalars missing");
+    auto m = std::make_unique<exa_model>();
+    m->m.vertices.assign(desc->vertices, desc->vertices + 4 * desc->numVertices);
+    m->m.indices.assign(desc->indices, desc->indices + 8 * desc->numElements);
+    m->m.gridlets.assign((const Gridlet *)desc->gridlets, (const Gridlet *)desc->gridlets + desc->numGridlets);
+    m->m.gridletScalars.assign(desc->gridletScalars, desc->gridletScalars + desc->numGridletScalars);
+    finalizeModel(m->m);
+    *out = m.release();
+  });
+}
+int exa_model_create_from_dual_mesh(const exa_dual_mesh *d, exa_model **out) {
+  return guarded([&] {
+    require(d && out, "invalid arguments");
+    auto m = std::make_unique<exa_model>();
+    m->m.vertices = d->mesh.vertices;
+    m->m.indices = d->mesh.indices;
+    m->m.gridlets = d->mesh.gridlets;
+    m->m.gridletScalars = d->mesh.gridletScalars;
+    finalizeModel(m->m);
+    *out = m.release();
+  });
+}
+int exa_model_get_bounds(const exa_model *m, float cb[6], float vr[2]) {
+  return guarded([&] {
+    require(m, "model is NULL");
+    if (cb) memcpy(cb, m->m.cellBounds, sizeof(float) * 6);
+    if (vr) memcpy(vr, m->m.valueRange, sizeof(float) * 2);
+  });
+}
+int exa_model_get_counts(const exa_model *m, int64_t c[4]) {
+  return guarded([&] {
+    require(m && c, "invalid arguments");
+    c[0] = (int64_t)m->m.vertices.size() / 4;
+    c[1] = m->m.numElements();
+    c[2] = (int64_t)m->m.gridlets.size();
+    c[3] = (int64_t)m->m.gridletScalars.size();
+  });
+}
+int exa_model_set_num_grid_cells(exa_model *m, const int32_t numMCs[3]) {
+  return guarded([&] {
+    require(m && numMCs, "invalid arguments");
+    require(numMCs[0] >= 0 && numMCs[1] >= 0 && numMCs[2] >= 0, "negative macrocell count");
+    require((int64_t)numMCs[0] * numMCs[1] * numMCs[2] < (1ll << 31), "macrocell grid too large");
+    for (int k = 0; k < 3; ++k) m->m.numMCs[k] = numMCs[k];
+  });
+}
+// Model.cpp:35-48 voxelSpaceTransform = T(from.lo)S(from.span) * rcp(T(to.lo)S(to.span))
+static void voxelTransform(const float from[6], const float to[6], float out[12]) {
+  for (int k = 0; k < 9; ++k) out[k] = 0.f;
+  for (int k = 0; k < 3; ++k) {
+    const float sf = from[3 + k] - from[k], st = to[3 + k] - to[k];
+    require(st != 0.f, "degenerate remap_to box");
+    const float inv = 1.f / st;
+    out[4 * k] = sf * inv;
+    out[9 + k] = sf * (-(inv * to[k])) + from[k];
+  }
+}
+int exa_model_set_voxel_space_transform(exa_model *m, const float from[6], const float to[6]) {
+  return guarded([&] {
+    require(m && from && to, "invalid arguments");
+    voxelTransform(from, to, m->m.voxelXfm);
+  });
+}
+void exa_model_destroy(exa_model *m) { delete m; }
+
+// ---- sampler ---------------------------------------------------------------
+int exa_sampler_create(exa_sampler **out) {
+  return guarded([&] {
+    require(out, "out is NULL");
+    *out = new exa_sampler();
+  });
+}
+int exa_sampler_build(exa_sampler *s, const exa_model *m, void *stream) {
+  return guarded([&] {
+    require(s && m, "invalid arguments");
+    s->build(m->m, (hipStream_t)stream);
+  });
+}
+int exa_sampler_compute_max_opacities(exa_sampler *s, const float *rgba, int n, float lo, float hi,
+                                      void *stream) {
+  return guarded([&] {
+    require(s, "sampler is NULL");
+    s->computeMaxOpacities(rgba, n, lo, hi, (hipStream_t)stream);
+  });
+}
+int exa_sampler_locate(const exa_sampler *s, const float *d_pos, int64_t n, exa_sample *d_out, void *stream) {
+  return guarded([&] {
+    require(s && (n == 0 || (d_pos && d_out)) && n >= 0, "invalid arguments");
+    if (!s->built) throw Status(EXA_ERR_STATE, "sampler not built");
+    launchLocate(s->view(), d_pos, n, (Sample *)d_out, nullptr, (hipStream_t)stream);
+    HIP_CHECK(hipGetLastError());
+  });
+}
+int exa_sampler_locate_counted(const exa_sampler *s, const float *d_pos, int64_t n, exa_sample *d_out,
+                               uint64_t counters[6], void *stream) {
+  return guarded([&] {
+    require(s && counters && (n == 0 || (d_pos && d_out)) && n >= 0, "invalid arguments");
+    if (!s->built) throw Status(EXA_ERR_STATE, "sampler not built");
+    hipStream_t st = (hipStream_t)stream;
+    HIP_CHECK(hipMemsetAsync(s->counters.ptr, 0, kNumCounters * sizeof(unsigned long long), st));
+    launchLocate(s->view(), d_pos, n, (Sample *)d_out, s->counters.ptr, st);
+    HIP_CHECK(hipGetLastError());
+    unsigned long long h[kNumCounters];
+    HIP_CHECK(hipMemcpyAsync(h, s->counters.ptr, sizeof(h), hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    for (int i = 0; i < 6; ++i) counters[i] = h[i];
+  });
+}
+int exa_sampler_test_sample(const exa_sampler *s, const float *d_pos, const int32_t *d_prim, int64_t n,
+                            exa_sample *d_out, void *stream) {
+  return guarded([&] {
+    require(s && (n == 0 || (d_pos && d_prim && d_out)) && n >= 0, "invalid arguments");
+    if (!s->built) throw Status(EXA_ERR_STATE, "sampler not built");
+    launchTestSample(s->view(), d_pos, d_prim, n, (Sample *)d_out, (hipStream_t)stream);
+    HIP_CHECK(hipGetLastError());
+  });
+}
+int exa_sampler_get_grid_info(const exa_sampler *s, int32_t dims[3], float bounds[6]) {
+  return guarded([&] {
+    require(s, "sampler is NULL");
+    if (!s->built) throw Status(EXA_ERR_STATE, "sampler not built");
+    if (dims) memcpy(dims, s->dims, sizeof(s->dims));
+    if (bounds) memcpy(bounds, s->bounds, sizeof(s->bounds));
+  });
+}
+int exa_sampler_get_value_ranges(const exa_sampler *s, float *out) {
+  return guarded([&] {
+    require(s && s->built, "sampler not built");
+    download(s->valueRanges, out);
+  });
+}
+int exa_sampler_get_max_opacities(const exa_sampler *s, float *out) {
+  return guarded([&] {
+    require(s && s->built, "sampler not built");
+    download(s->maxOpacities, out);
+  });
+}
+int exa_sampler_get_gridlet_value_ranges(const exa_sampler *s, float *out) {
+  return guarded([&] {
+    require(s && s->built, "sampler not built");
+    download(s->gridletRanges, out);
+  });
+}
+int exa_sampler_get_prim_max_opacities(const exa_sampler *s, float *g, float *e) {
+  return guarded([&] {
+    require(s && s->built, "sampler not built");
+    if (!s->haveMaxOp) throw Status(EXA_ERR_STATE, "max opacities not computed");
+    download(s->gridletMaxOp, g);
+    download(s->elemMaxOp, e);
+  });
+}
+int exa_sampler_get_info(const exa_sampler *s, int64_t info[8]) {
+  return guarded([&] {
+    require(s && info, "invalid arguments");
+    info[0] = (int64_t)s->gNodes.count;
+    info[1] = s->gDepth;
+    info[2] = (int64_t)s->gRefs.count;
+    info[3] = (int64_t)s->eNodes.count;
+    info[4] = s->eDepth;
+    info[5] = (int64_t)s->eRefs.count;
+    info[6] = (int64_t)(s->vertices.bytes() + s->indices.bytes() + s->gridlets.bytes() + s->gridletScalars.bytes());
+    info[7] = (int64_t)(s->gNodes.bytes() + s->eNodes.bytes() + s->gRefs.bytes() + s->eRefs.bytes());
+  });
+}
+void exa_sampler_destroy(exa_sampler *s) { delete s; }
+
+// ---- renderer --------------------------------------------------------------
+int exa_renderer_create(const exa_model *m, const int32_t numMCs[3], const float remapFrom[6],
+                        const float remapTo[6], exa_renderer **out) {
+  return guarded([&] {
+    require(m && numMCs && out, "invalid arguments");
+    auto r = std::make_unique<exa_renderer>();
+    r->model = m->m;
+    HostModel &M = r->model;
+    // OWLRenderer.cpp:166-181
+    float cellSize[3] = {M.cellBounds[3] - M.cellBounds[0], M.cellBounds[4] - M.cellBounds[1],
+                         M.cellBounds[5] - M.cellBounds[2]};
+    float lightScale = 1.f;
+    while (fmaxf(fmaxf(cellSize[0], cellSize[1]), cellSize[2]) > 1000.f) {
+      for (int k = 0; k < 3; ++k) cellSize[k] /= 1000.f;
+      lightScale /= 1000.f;
+    }
+    static const float unit[6] = {0, 0, 0, 1, 1, 1};
+    voxelTransform(remapFrom ? remapFrom : unit, remapTo ? remapTo : unit, M.voxelXfm);
+    for (int k = 0; k < 3; ++k) M.numMCs[k] = numMCs[k];
+    RenderParams &P = r->P;
+    memset(&P, 0, sizeof(P));
+    memcpy(P.voxelXfm, M.voxelXfm, sizeof(P.voxelXfm));
+    affineInverse(M.voxelXfm, P.voxelXfmInv);
+    // Model::getBounds (Model.cpp:50-70)
+    xfmPointHost(P.voxelXfmInv, M.cellBounds, r->modelBounds);
+    xfmPointHost(P.voxelXfmInv, M.cellBounds + 3, r->modelBounds + 3);
+    memcpy(r->valueRange, M.valueRange, sizeof(r->valueRange));
+    affineIdentity(P.lightXfm);
+    P.lightXfm[0] = P.lightXfm[4] = P.lightXfm[8] = lightScale;
+    // initGPU (OWLRenderer.cpp:309-582)
+    r->sampler.build(M, 0);
+    memcpy(P.worldBounds, r->modelBounds, sizeof(P.worldBounds));
+    P.clipEnabled = 0;
+    P.clipN[0] = 0.f;
+    P.clipN[1] = 0.f;
+    P.clipN[2] = 1.f;
+    P.clipD = 0.5f * (r->modelBounds[2] + r->modelBounds[5]);
+    P.subImage[0] = 0.f;
+    P.subImage[1] = 0.f;
+    P.subImage[2] = 0.f;
+    P.subImage[3] = 1.f;
+    P.subImageSel[2] = 1.f;
+    P.subImageSel[3] = 1.f;
+    P.integrator = 0;
+    P.shadeMode = 0;
+    P.maxBounces = 1024;
+    P.debugCrosshair = 1;
+    P.opacityScale = 1.f;
+    P.spp = 1;
+    *out = r.release();
+  });
+}
+int exa_renderer_set_camera(exa_renderer *r, const float org[3], const float d00[3], const float du[3],
+                            const float dv[3]) {
+  return guarded([&] {
+    require(r && org && d00 && du && dv, "invalid arguments");
+    memcpy(r->P.camOrg, org, 12);
+    memcpy(r->P.camDir00, d00, 12);
+    memcpy(r->P.camDu, du, 12);
+    memcpy(r->P.camDv, dv, 12);
+  });
+}
+int exa_renderer_resize(exa_renderer *r, int w, int h) {
+  return guarded([&] {
+    require(r && w > 0 && h > 0 && (int64_t)w * h < (1ll << 31), "invalid size");
+    if (w != r->P.W || h != r->P.H) {
+      r->accum.alloc((size_t)w * h);
+      r->P.W = w;
+      r->P.H = h;
+    }
+  });
+}
+static void doRender(exa_renderer *r, uint32_t *fb, unsigned long long *counters, hipStream_t s) {
+  if (r->P.W <= 0 || r->P.H <= 0) throw Status(EXA_ERR_STATE, "resize() not called");
+  if (r->tf.count == 0) throw Status(EXA_ERR_STATE, "setColorMap() not called");
+  RenderParams P = r->P;
+  P.accum = r->accum.ptr;
+  P.fb = fb;
+  P.accumID = r->accumID;
+  P.spp = r->spp < 1 ? 1 : r->spp;
+  P.tf = r->tf.ptr;
+  P.tfN = (int32_t)r->tf.count;
+  launchRender(P, r->sampler.view(), r->sampler.gridView(), counters, s);
+  HIP_CHECK(hipGetLastError());
+  r->accumID++;
+}
+int exa_renderer_render(exa_renderer *r, uint32_t *d_fb, void *stream) {
+  return guarded([&] {
+    require(r, "renderer is NULL");
+    doRender(r, d_fb, nullptr, (hipStream_t)stream);
+  });
+}
+int exa_renderer_render_counted(exa_renderer *r, uint32_t *d_fb, uint64_t counters[10], void *stream) {
+  return guarded([&] {
+    require(r && counters, "invalid arguments");
+    hipStream_t st = (hipStream_t)stream;
+    unsigned long long *dc = r->sampler.counters.ptr;
+    HIP_CHECK(hipMemsetAsync(dc, 0, kNumCounters * sizeof(unsigned long long), st));
+    doRender(r, d_fb, dc, st);
+    unsigned long long h[kNumCounters];
+    HIP_CHECK(hipMemcpyAsync(h, dc, sizeof(h), hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    for (int i = 0; i < kNumCounters; ++i) counters[i] = h[i];
+  });
+}
+int exa_renderer_get_accum(const exa_renderer *r, float *out) {
+  return guarded([&] {
+    require(r && out, "invalid arguments");
+    download(r->accum, out);
+  });
+}
+int exa_renderer_set_type(exa_renderer *r, int type) {
+  return guarded([&] {
+    require(r, "renderer is NULL");
+    require(type == 0 || type == 1, "supported integrators: 0 path tracer, 1 direct lighting");
+    r->P.integrator = type;
+    r->accumID = 0;
+  });
+}
+int exa_renderer_set_color_map(exa_renderer *r, const float *rgba, int n) {
+  return guarded([&] {
+    require(r && rgba && n > 0, "invalid color map");
+    r->colorMap.assign(rgba, rgba + 4 * (size_t)n);
+    r->tf.upload((const float4 *)rgba, n, 0);
+    r->updateMaxOpacities();
+    r->accumID = 0;
+  });
+}
+int exa_renderer_set_range(exa_renderer *r, float lo, float hi) {
+  return guarded([&] {
+    require(r, "renderer is NULL");
+    r->absDomain[0] = lo;
+    r->absDomain[1] = hi;
+    // OWLRenderer.cpp:717-733
+    r->P.tfDomain[0] = r->absDomain[0] + (r->relDomain[0] / 100.f) * (r->absDomain[1] - r->absDomain[0]);
+    r->P.tfDomain[1] = r->absDomain[0] + (r->relDomain[1] / 100.f) * (r->absDomain[1] - r->absDomain[0]);
+    r->updateMaxOpacities();
+  });
+}
+int exa_renderer_set_rel_domain(exa_renderer *r, float lo, float hi) {
+  if (!r) return EXA_ERR_INVALID;
+  r->relDomain[0] = lo;
+  r->relDomain[1] = hi;
+  return exa_renderer_set_range(r, r->absDomain[0], r->absDomain[1]);
+}
+int exa_renderer_set_opacity_scale(exa_renderer *r, float scale) {
+  return guarded([&] {
+    require(r, "renderer is NULL");
+    r->P.opacityScale = powf(1.1f, scale - 100);  // OWLRenderer.cpp:741-744
+  });
+}
+int exa_renderer_set_clip_plane(exa_renderer *r, int id, int enabled, const float N[3], float d) {
+  return guarded([&] {
+    require(r && N, "invalid arguments");
+    require(id == 0, "clip plane id must be 0 (CLIP_PLANES_MAX == 1)");
+    r->P.clipEnabled = enabled ? 1 : 0;
+    memcpy(r->P.clipN, N, 12);
+    r->P.clipD = d;
+    r->accumID = 0;
+  });
+}
+int exa_renderer_set_shade_mode(exa_renderer *r, int mode) {
+  return guarded([&] {
+    require(r, "renderer is NULL");
+    require(mode == 0, "only the default shade mode is implemented");
+    r->P.shadeMode = mode;
+    r->accumID = 0;
+  });
+}
+int exa_renderer_set_light_source(exa_renderer *r, int id, const float pos[3], float intensity, int on) {
+  return guarded([&] {
+    require(r && pos && id >= 0 && id < kLightsMax, "invalid light");
+    LightParams &L = r->P.lights[id];
+    memcpy(L.pos, pos, 12);
+    L.intensity = intensity;
+    L.on = on ? 1 : 0;
+    L.type = 0;
+    r->accumID = 0;
+  });
+}
+int exa_renderer_set_directional_light(exa_renderer *r, int id, const float dir[3], float intensity, int on) {
+  return guarded([&] {
+    require(r && dir && id >= 0 && id < kLightsMax, "invalid light");
+    LightParams &L = r->P.lights[id];
+    memcpy(L.pos, dir, 12);
+    L.intensity = intensity;
+    L.on = on ? 1 : 0;
+    L.type = 1;
+    r->accumID = 0;
+  });
+}
+int exa_renderer_set_light_space_transform(exa_renderer *r, const float xfm[12]) {
+  return guarded([&] {
+    require(r && xfm, "invalid arguments");
+    memcpy(r->P.lightXfm, xfm, sizeof(r->P.lightXfm));
+    r->accumID = 0;
+  });
+}
+int exa_renderer_set_sub_image(exa_renderer *r, const float box[4], int active) {
+  return guarded([&] {
+    require(r && box, "invalid arguments");
+    memcpy(r->P.subImage, box, 16);
+    r->P.subImageActive = active ? 1 : 0;
+    r->accumID = 0;
+  });
+}
+int exa_renderer_set_sub_image_selection(exa_renderer *r, const float box[4], int active) {
+  return guarded([&] {
+    require(r && box, "invalid arguments");
+    memcpy(r->P.subImageSel, box, 16);
+    r->P.subImageSelecting = active ? 1 : 0;
+    r->accumID = 0;
+  });
+}
+int exa_renderer_reset_accum(exa_renderer *r) {
+  if (!r) return EXA_ERR_INVALID;
+  r->accumID = 0;
+  return EXA_OK;
+}
+int exa_renderer_set_spp(exa_renderer *r, int spp) {
+  if (!r) return EXA_ERR_INVALID;
+  r->spp = spp;
+  return EXA_OK;
+}
+int exa_renderer_set_max_bounces(exa_renderer *r, int n) {
+  if (!r || n < 0) return EXA_ERR_INVALID;
+  r->P.maxBounces = n;
+  r->accumID = 0;
+  return EXA_OK;
+}
+int exa_renderer_set_debug_crosshair(exa_renderer *r, int on) {
+  if (!r) return EXA_ERR_INVALID;
+  r->P.debugCrosshair = on ? 1 : 0;
+  return EXA_OK;
+}
+int exa_renderer_get_bounds(const exa_renderer *r, float b[6]) {
+  if (!r || !b) return EXA_ERR_INVALID;
+  memcpy(b, r->modelBounds, sizeof(r->modelBounds));
+  return EXA_OK;
+}
+int exa_renderer_get_value_range(const exa_renderer *r, float v[2]) {
+  if (!r || !v) return EXA_ERR_INVALID;
+  memcpy(v, r->valueRange, sizeof(r->valueRange));
+  return EXA_OK;
+}
+int exa_renderer_get_accum_id(const exa_renderer *r) { return r ? r->accumID : EXA_ERR_INVALID; }
+const exa_sampler *exa_renderer_get_sampler(const exa_renderer *r) { return r ? &r->sampler : nullptr; }
+int exa_renderer_get_state(const exa_renderer *r, exa_render_state *o) {
+  return guarded([&] {
+    require(r && o, "invalid arguments");
+    const RenderParams &P = r->P;
+    memset(o, 0, sizeof(*o));
+    o->W = P.W;
+    o->H = P.H;
+    o->spp = r->spp < 1 ? 1 : r->spp;
+    o->accumID = r->accumID;
+    o->integrator = P.integrator;
+    o->shadeMode = P.shadeMode;
+    o->maxBounces = P.maxBounces;
+    o->debugCrosshair = P.debugCrosshair;
+    memcpy(o->camOrg, P.camOrg, 12);
+    memcpy(o->camDir00, P.camDir00, 12);
+    memcpy(o->camDu, P.camDu, 12);
+    memcpy(o->camDv, P.camDv, 12);
+    memcpy(o->worldBounds, P.worldBounds, 24);
+    memcpy(o->voxelXfm, P.voxelXfm, 48);
+    memcpy(o->voxelXfmInv, P.voxelXfmInv, 48);
+    memcpy(o->lightXfm, P.lightXfm, 48);
+    memcpy(o->gridDims, r->sampler.dims, 12);
+    memcpy(o->gridBounds, r->sampler.bounds, 24);
+    memcpy(o->tfDomain, P.tfDomain, 8);
+    o->opacityScale = P.opacityScale;
+    o->tfN = (int32_t)r->tf.count;
+    o->clipEnabled = P.clipEnabled;
+    memcpy(o->clipN, P.clipN, 12);
+    o->clipD = P.clipD;
+    for (int i = 0; i < 4; ++i) {
+      memcpy(o->lightPos[i], P.lights[i].pos, 12);
+      o->lightIntensity[i] = P.lights[i].intensity;
+      o->lightOn[i] = P.lights[i].on;
+      o->lightType[i] = P.lights[i].type;
+    }
+  });
+}
+void exa_renderer_destroy(exa_renderer *r) { delete r; }
+
+// ---- helpers ---------------------------------------------------------------
+int exa_camera_look_at(const float from[3], const float at[3], const float up[3], float fovyDeg, int W, int H,
+                       float org[3], float dir00[3], float du[3], float dv[3]) {
+  return guarded([&] {
+    require(from && at && up && org && dir00 && du && dv && W > 0 && H > 0, "invalid arguments");
+    auto sub = [](const float *a, const float *b, float *o) { for (int k = 0; k < 3; ++k) o[k] = a[k] - b[k]; };
+    auto cross3 = [](const float *a, const float *b, float *o) {
+      o[0] = a[1] * b[2] - a[2] * b[1];
+      o[1] = a[2] * b[0] - a[0] * b[2];
+      o[2] = a[0] * b[1] - a[1] * b[0];
+    };
+    auto norm = [](float *v) {
+      const float l = sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+      require(l > 0.f, "degenerate camera frame");
+      for (int k = 0; k < 3; ++k) v[k] /= l;
+    };
+    // camera frame: vz points from the point of interest to the eye
+    float vz[3], vx[3], vy[3], d[3];
+    sub(from, at, vz);
+    const float focal = sqrtf(vz[0] * vz[0] + vz[1] * vz[1] + vz[2] * vz[2]);
+    norm(vz);
+    cross3(up, vz, vx);
+    norm(vx);
+    cross3(vz, vx, vy);
+    norm(vy);
+    (void)d;
+    // viewer.cpp:79-107 SimpleCamera
+    auto eps = [](float f) { return fabsf(f) * (float)(1. / (1 << 21)); };
+    const float minFocal = fmaxf(fmaxf(fmaxf(eps(from[0]), eps(from[1])), eps(from[2])),
+                                 fmaxf(fmaxf(eps(vx[0]), eps(vx[1])), eps(vx[2])));
+    const float fd = fmaxf(minFocal, focal);
+    const float aspect = W / (float)H;
+    const float screen_height = 2.f * tanf(fovyDeg / 2.f * (float)M_PI / 180.f) * fd;
+    float vert[3], horiz[3];
+    for (int k = 0; k < 3; ++k) {
+      vert[k] = screen_height * vy[k];
+      horiz[k] = screen_height * aspect * vx[k];
+    }
+    for (int k = 0; k < 3; ++k) {
+      org[k] = from[k];
+      dir00[k] = -fd * vz[k] - 0.5f * vert[k] - 0.5f * horiz[k];
+      // viewer.cpp:550-560 cameraChanged
+      du[k] = horiz[k] / (float)W;
+      dv[k] = vert[k] / (float)H;
+    }
+  });
+}
+
+int exa_xf_load(const char *path, float *opacityScale, float absDomain[2], float relDomain[2], int *numColors,
+                float *rgba, int maxColors) {
+  return guarded([&] {
+    require(path, "path is NULL");
+    std::ifstream in(path, std::ios::binary);
+    if (!in.good()) throw Status(EXA_ERR_INVALID, std::string("cannot open ") + path);
+    uint64_t magic = 0;
+    in.read((char *)&magic, 8);
+    if (magic != 0x1235abc000ull) throw Status(EXA_ERR_INVALID, std::string(path) + ": not a .xf file");
+    float os, ad[2], rd[2];
+    int32_t n = 0;
+    in.read((char *)&os, 4);
+    in.read((char *)ad, 8);
+    in.read((char *)rd, 8);
+    in.read((char *)&n, 4);
+    if (!in.good() || n <= 0 || n > (1 << 20)) throw Status(EXA_ERR_INVALID, "truncated .xf header");
+    std::vector<float> cm(4 * (size_t)n);
+    in.read((char *)cm.data(), cm.size() * 4);
+    if (!in.good()) throw Status(EXA_ERR_INVALID, "truncated .xf color map");
+    if (opacityScale) *opacityScale = os;
+    if (absDomain) memcpy(absDomain, ad, 8);
+    if (relDomain) memcpy(relDomain, rd, 8);
+    if (numColors) *numColors = n;
+    if (rgba) {
+      require(maxColors >= n, "rgba buffer too small");
+      memcpy(rgba, cm.data(), cm.size() * 4);
+    }
+  });
+}
+
+int exa_tf_rasterize(const float *pts, int np, int N, float *rgba) {
+  return guarded([&] {
+    require(pts && rgba && np >= 1 && N >= 2, "invalid arguments");
+    for (int i = 1; i < np; ++i) require(pts[5 * i] >= pts[5 * (i - 1)], "control points must ascend in x");
+    for (int i = 0; i < N; ++i) {
+      const float x = (float)i / (float)(N - 1);
+      int k = 0;
+      while (k + 1 < np && pts[5 * (k + 1)] < x) ++k;
+      const float *a = pts + 5 * k;
+      const float *b = pts + 5 * (k + 1 < np ? k + 1 : k);
+      float t = 0.f;
+      if (x <= a[0]) t = 0.f;
+      else if (b[0] > a[0]) t = fminf(1.f, (x - a[0]) / (b[0] - a[0]));
+      for (int c = 0; c < 4; ++c) rgba[4 * i + c] = a[1 + c] + t * (b[1 + c] - a[1 + c]);
+    }
+  });
+}
+
+}  // extern "C"

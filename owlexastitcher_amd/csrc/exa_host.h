@@ -1,0 +1,101 @@
+// exa_host.h -- host-side builders of the exaStitch path (no HIP runtime):
+// synthetic AMR scenes, the dual-mesh/gridlet generator that replaces the
+// reference's offline amrMakeDualMesh + amrMakeGrids tools, the in-memory
+// ExaStitchModel, and the point-location BVH.
+#pragma once
+#include <stdint.h>
+#include <string>
+#include <vector>
+#include "exa_types.h"
+
+namespace exa {
+
+int hostThreads();
+
+// ---------------------------------------------------------------------------
+// synthetic AMR scenes
+// ---------------------------------------------------------------------------
+struct AmrScene {
+  std::vector<AmrCell> cells;
+  std::vector<float> scalars;  // one per cell, input order == scalar ID
+  std::vector<float> params;   // generator parameters (for tests)
+};
+
+// Two-level Gaussian scene: coarse cells of width 2^coarseLevel on a
+// baseRes^3 grid; coarse cells lying inside [fineLo,fineHi)^3 are split once.
+// Scalar at each cell centre = sum of numGaussians isotropic Gaussians
+// (centre U[0,domain]^3, sigma U[sigLo,sigHi], amplitude U[ampLo,ampHi]).
+void amrGaussians(AmrScene &out, int baseRes, int coarseLevel, int fineLo, int fineHi,
+                  int numGaussians, uint64_t seed);
+
+// Three-level fractal-noise scene: base cells of width 2^baseLevel on a
+// baseRes^3 grid; a cell is split where the noise at its centre exceeds
+// thresh1 (next level) and thresh2 (the level after); a vertex 2:1 balance
+// pass follows.  Scalar = noise * radial falloff.
+void amrNoise(AmrScene &out, int baseRes, int baseLevel, float thresh1, float thresh2,
+              int octaves, float contrast, uint64_t seed);
+
+// ---------------------------------------------------------------------------
+// dual mesh (tools/amrMakeDualMesh/makeDual.cpp, makeGrids.cpp)
+// ---------------------------------------------------------------------------
+struct DualMesh {
+  std::vector<float> vertices;     // float4 per vertex (cell centre, scalar)
+  std::vector<int32_t> indices;    // int[8] per element, -1 padded
+  std::vector<Gridlet> gridlets;
+  std::vector<float> gridletScalars;
+  int64_t counts[4] = {0, 0, 0, 0};  // tets, pyramids, wedges, hexes
+  int64_t numCubes = 0;
+  int64_t conflicts = 0;   // brick scalar write conflicts + impossible cases
+  int64_t degenerate = 0;  // dual cells dropped as degenerate
+};
+// Throws std::runtime_error on invalid input.
+void buildDualMesh(const AmrCell *cells, const float *scalars, int64_t n, DualMesh &out);
+
+// ---------------------------------------------------------------------------
+// model (model/ExaStitchModel.cpp:24-257 in-memory form)
+// ---------------------------------------------------------------------------
+struct HostModel {
+  std::vector<float> vertices;
+  std::vector<int32_t> indices;
+  std::vector<Gridlet> gridlets;
+  std::vector<float> gridletScalars;
+  float cellBounds[6];
+  float valueRange[2];
+  float voxelXfm[12];      // Model::voxelSpaceTransform (identity by default)
+  int32_t numMCs[3] = {0, 0, 0};
+  int64_t numElements() const { return (int64_t)indices.size() / 8; }
+};
+// Validates element rows and gridlet records; computes cellBounds and
+// valueRange as ExaStitchModel::load does.  Throws on invalid input.
+void finalizeModel(HostModel &m);
+
+// ---------------------------------------------------------------------------
+// point-location BVH (replaces the OptiX user-geometry BLAS/TLAS)
+// ---------------------------------------------------------------------------
+// 64-byte binary node holding both children's boxes.  A child ref >= 0 is an
+// inner node index; a child with count > 0 is a leaf covering
+// primRefs[ref .. ref+count); count == 0 marks an inner node.  An empty child
+// has count == -1 and an inverted
+// box.
+struct BvhNode {
+  float lo0[3], hi0[3];
+  float lo1[3], hi1[3];
+  int32_t ref0, ref1;
+  int32_t cnt0, cnt1;
+};
+static_assert(sizeof(BvhNode) == 64, "BvhNode must stay 64 bytes");
+
+struct Bvh {
+  std::vector<BvhNode> nodes;
+  std::vector<int32_t> primRefs;  // ascending within each leaf
+  int depth = 0;
+};
+constexpr int kBvhMaxDepth = 32;
+// boxes: 6 floats (lo xyz, hi xyz) per primitive.
+void buildBvh(const float *boxes, int64_t n, int leafSize, Bvh &out);
+
+void gridletBoxes(const std::vector<Gridlet> &g, std::vector<float> &boxes);
+void elementBoxes(const std::vector<int32_t> &indices, const std::vector<float> &vertices,
+                  std::vector<float> &boxes);
+
+}  // namespace exa

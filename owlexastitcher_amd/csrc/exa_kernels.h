@@ -1,0 +1,110 @@
+// exa_kernels.h -- device views and kernel launch wrappers shared by the
+// sampler, renderer and C-ABI translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "exa_device.h"
+#include "exa_host.h"
+
+namespace exa {
+
+// Point-location data on the device (the ExaStitchSampler::LP of
+// sampler/ExaStitchSampler.h:42-49 plus the geometry the OptiX programs
+// read through their SBT records, programs/ExaStitchSampler.cu:25-35).
+struct SamplerView {
+  const Gridlet *gridlets;
+  const float *gridletScalars;
+  const float *gridletMaxOp;  // nullptr: no culling
+  const BvhNode *gNodes;
+  const int32_t *gRefs;
+  int32_t numGridlets;
+  const float4 *vertices;
+  const int32_t *indices;
+  const float *elemMaxOp;  // nullptr: no culling
+  const BvhNode *eNodes;
+  const int32_t *eRefs;
+  int32_t numElements;
+};
+
+// GridTraversable (Grid.cuh:21-24) plus the majorant buffer.
+struct GridView {
+  int32_t dims[3];
+  float bounds[6];
+  const float *majorants;
+};
+
+constexpr int kLocateBlock = 256;
+constexpr int kStackDepth = kBvhMaxDepth;
+
+// Per-launch counters of the instrumented render/locate variants; the
+// algorithmic-byte roofline is computed from these.
+enum Counter {
+  kCntGridletHit = 0,  // full point location ending in a gridlet
+  kCntTetHit,
+  kCntPyrHit,
+  kCntWedgeHit,
+  kCntHexHit,
+  kCntMiss,
+  kCntCachedHit,  // testSample hit on the remembered gridlet
+  kCntDdaCells,
+  kCntPixels,
+  kCntTfFetch,
+  kNumCounters
+};
+
+// ---- sampler kernels (sampler.hip) ----------------------------------------
+void launchGridletValueRanges(const Gridlet *g, const float *scalars, int64_t n, float2 *ranges,
+                              hipStream_t s);
+void launchGridBuild(const SamplerView &sv, const int32_t dims[3], const float bounds[6],
+                     float2 *ranges, uint2 *scratch, hipStream_t s);
+void launchMapMaxOpacity(const float2 *ranges, int64_t n, const float4 *cmap, int numColors,
+                         float x0, float x1, float *out, hipStream_t s);
+void launchElementMaxOpacity(const float4 *vertices, const int32_t *indices, int64_t n,
+                             const float4 *cmap, int numColors, float x0, float x1, float *out,
+                             hipStream_t s);
+void launchLocate(const SamplerView &sv, const float *pos, int64_t n, Sample *out,
+                  unsigned long long *counters, hipStream_t s);
+void launchTestSample(const SamplerView &sv, const float *pos, const int32_t *primIDs, int64_t n,
+                      Sample *out, hipStream_t s);
+
+// ---- renderer (render.hip) ------------------------------------------------
+constexpr int kLightsMax = 4;  // common.h:27 LIGHTS_MAX
+struct LightParams {
+  float pos[3];
+  float intensity;
+  int32_t on;
+  int32_t type;  // 0 point, 1 directional
+};
+
+// LaunchParams (LaunchParams.h:56-119) restricted to the ExaStitch path.
+struct RenderParams {
+  int32_t W, H, spp, accumID;
+  int32_t integrator;  // 0 path tracer, 1 direct lighting
+  int32_t shadeMode;   // 0 default
+  int32_t maxBounces;
+  int32_t debugCrosshair;
+  float camOrg[3], camDir00[3], camDu[3], camDv[3];
+  float worldBounds[6];
+  float voxelXfm[12];
+  float voxelXfmInv[12];
+  float lightXfm[12];
+  float tfDomain[2];
+  float opacityScale;
+  int32_t tfN;
+  const float4 *tf;
+  int32_t clipEnabled;
+  float clipN[3];
+  float clipD;
+  int32_t subImageActive;
+  float subImage[4];  // lower.xy, upper.xy
+  int32_t subImageSelecting;
+  float subImageSel[4];
+  LightParams lights[kLightsMax];
+  float4 *accum;
+  uint32_t *fb;
+};
+
+void launchRender(const RenderParams &p, const SamplerView &sv, const GridView &gv,
+                  unsigned long long *counters, hipStream_t s);
+
+}  // namespace exa

@@ -1,0 +1,384 @@
+"""Python host mirror of the reference's exaStitch interfaces over the C ABI.
+
+Class and method names follow the reference so scripts read like its own
+host code:
+  ExaStitchModel   model/ExaStitchModel.h:24-57 + model/Model.h:29-65
+  ExaStitchSampler sampler/Sampler.h:28-69, sampler/ExaStitchSampler.h:37-106
+  OWLRenderer      OWLRenderer.h:29-175
+Device buffers are torch tensors on the current HIP device (plumbing only);
+every computation runs in libexastitch.so.
+"""
+import ctypes as C
+
+import numpy as np
+
+from ._capi import (AmrCell, Gridlet, RenderState, StitchModelDesc, check, lib)
+
+PATH_TRACER = 0
+DIRECT_LIGHTING = 1
+
+
+def _ptr(a):
+    return C.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def _f32(x, n=None):
+    a = np.ascontiguousarray(np.asarray(x, dtype=np.float32))
+    if n is not None and a.size != n:
+        raise ValueError(f"expected {n} floats, got {a.size}")
+    return a
+
+
+class AmrScene:
+    """Leaf cells (position, level) + per-cell scalars of an AMR hierarchy."""
+
+    def __init__(self, handle):
+        self.h = handle
+
+    @classmethod
+    def gaussians(cls, base_res=64, coarse_level=1, fine_lo=32, fine_hi=96, num_gaussians=8, seed=1):
+        h = C.c_void_p()
+        check(lib().exa_amr_create_gaussians(base_res, coarse_level, fine_lo, fine_hi, num_gaussians, seed,
+                                             C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def noise(cls, base_res=128, base_level=2, thresh1=0.55, thresh2=0.7, octaves=6, contrast=2.2, seed=1):
+        h = C.c_void_p()
+        check(lib().exa_amr_create_noise(base_res, base_level, thresh1, thresh2, octaves, contrast, seed,
+                                         C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def from_cells(cls, cells, scalars):
+        cells = np.ascontiguousarray(cells, dtype=np.int32).reshape(-1, 4)
+        scalars = _f32(scalars)
+        h = C.c_void_p()
+        check(lib().exa_amr_create_from_cells(_ptr(cells), _ptr(scalars), len(cells), C.byref(h)))
+        return cls(h)
+
+    def __len__(self):
+        return int(lib().exa_amr_num_cells(self.h))
+
+    def arrays(self):
+        n = len(self)
+        cells = np.empty((n, 4), np.int32)
+        scalars = np.empty(n, np.float32)
+        check(lib().exa_amr_get(self.h, _ptr(cells), _ptr(scalars)))
+        return cells, scalars
+
+    def params(self):
+        n = lib().exa_amr_get_params(self.h, None, 0)
+        out = np.empty(max(n, 1), np.float32)
+        lib().exa_amr_get_params(self.h, _ptr(out), n)
+        return out[:n]
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().exa_amr_destroy(self.h)
+            self.h = None
+
+
+class DualMesh:
+    """Output of the dual-mesh generator (tools/amrMakeDualMesh)."""
+
+    COUNT_NAMES = ("tets", "pyramids", "wedges", "hexes", "elements", "vertices", "gridlets",
+                   "gridlet_scalars", "cubes", "conflicts", "degenerate")
+
+    def __init__(self, amr):
+        self.h = C.c_void_p()
+        check(lib().exa_dual_mesh_build(amr.h, C.byref(self.h)))
+
+    def counts(self):
+        c = np.zeros(11, np.int64)
+        check(lib().exa_dual_mesh_counts(self.h, _ptr(c)))
+        return dict(zip(self.COUNT_NAMES, (int(v) for v in c)))
+
+    def arrays(self):
+        c = self.counts()
+        idx = np.empty((c["elements"], 8), np.int32)
+        verts = np.empty((c["vertices"], 4), np.float32)
+        gl = (Gridlet * max(c["gridlets"], 1))()
+        gs = np.empty(c["gridlet_scalars"], np.float32)
+        check(lib().exa_dual_mesh_get(self.h, _ptr(idx), _ptr(verts), gl, _ptr(gs)))
+        gla = np.ctypeslib.as_array(gl).view(np.int32).reshape(-1, 8)[: c["gridlets"]].copy()
+        return idx, verts, gla, gs
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().exa_dual_mesh_destroy(self.h)
+            self.h = None
+
+
+class ExaStitchModel:
+    """ExaStitchModel: element vertices/indices + gridlets (copied in)."""
+
+    def __init__(self, handle):
+        self.h = handle
+
+    @classmethod
+    def from_dual_mesh(cls, dm):
+        h = C.c_void_p()
+        check(lib().exa_model_create_from_dual_mesh(dm.h, C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def from_arrays(cls, vertices, indices, gridlets, gridlet_scalars):
+        v = _f32(vertices).reshape(-1)
+        i = np.ascontiguousarray(indices, dtype=np.int32).reshape(-1)
+        g = np.ascontiguousarray(gridlets, dtype=np.int32).reshape(-1)
+        s = _f32(gridlet_scalars).reshape(-1)
+        d = StitchModelDesc(_ptr(v).value, v.size // 4, _ptr(i).value, i.size // 8, _ptr(g).value, g.size // 8,
+                            _ptr(s).value, s.size)
+        h = C.c_void_p()
+        check(lib().exa_model_create(C.byref(d), C.byref(h)))
+        return cls(h)
+
+    def bounds(self):
+        cb = np.zeros(6, np.float32)
+        vr = np.zeros(2, np.float32)
+        check(lib().exa_model_get_bounds(self.h, _ptr(cb), _ptr(vr)))
+        return cb, vr
+
+    def counts(self):
+        c = np.zeros(4, np.int64)
+        check(lib().exa_model_get_counts(self.h, _ptr(c)))
+        return dict(zip(("vertices", "elements", "gridlets", "gridlet_scalars"), (int(x) for x in c)))
+
+    def setNumGridCells(self, num_mcs):
+        d = np.ascontiguousarray(num_mcs, dtype=np.int32)
+        check(lib().exa_model_set_num_grid_cells(self.h, _ptr(d)))
+
+    def setVoxelSpaceTransform(self, remap_from, remap_to):
+        check(lib().exa_model_set_voxel_space_transform(self.h, _ptr(_f32(remap_from, 6)), _ptr(_f32(remap_to, 6))))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().exa_model_destroy(self.h)
+            self.h = None
+
+
+def _stream(stream):
+    if stream is None:
+        return None
+    return C.c_void_p(stream.cuda_stream if hasattr(stream, "cuda_stream") else int(stream))
+
+
+class ExaStitchSampler:
+    """ExaStitchSampler: build / computeMaxOpacities / device sample()."""
+
+    def __init__(self, handle=None, owner=None):
+        self._owned = handle is None
+        if handle is None:
+            handle = C.c_void_p()
+            check(lib().exa_sampler_create(C.byref(handle)))
+        self.h = handle
+        self._owner = owner
+
+    def build(self, model, stream=None):
+        check(lib().exa_sampler_build(self.h, model.h, _stream(stream)))
+        return True
+
+    def computeMaxOpacities(self, rgba, xf_range, stream=None):
+        cm = _f32(rgba).reshape(-1, 4)
+        check(lib().exa_sampler_compute_max_opacities(self.h, _ptr(cm), len(cm), float(xf_range[0]),
+                                                      float(xf_range[1]), _stream(stream)))
+
+    def locate(self, d_pos, d_out, stream=None):
+        """d_pos: float32 tensor (n,3) on the GPU; d_out: int32 tensor (n,3)
+        viewed as Sample records."""
+        n = d_pos.shape[0]
+        check(lib().exa_sampler_locate(self.h, C.c_void_p(d_pos.data_ptr()), n, C.c_void_p(d_out.data_ptr()),
+                                       _stream(stream)))
+
+    def locate_counted(self, d_pos, d_out, stream=None):
+        n = d_pos.shape[0]
+        c = np.zeros(6, np.uint64)
+        check(lib().exa_sampler_locate_counted(self.h, C.c_void_p(d_pos.data_ptr()), n,
+                                               C.c_void_p(d_out.data_ptr()), _ptr(c), _stream(stream)))
+        return dict(zip(("gridlet", "tet", "pyr", "wedge", "hex", "miss"), (int(x) for x in c)))
+
+    def testSample(self, d_pos, d_prim, d_out, stream=None):
+        n = d_pos.shape[0]
+        check(lib().exa_sampler_test_sample(self.h, C.c_void_p(d_pos.data_ptr()), C.c_void_p(d_prim.data_ptr()),
+                                            n, C.c_void_p(d_out.data_ptr()), _stream(stream)))
+
+    def grid_info(self):
+        dims = np.zeros(3, np.int32)
+        b = np.zeros(6, np.float32)
+        check(lib().exa_sampler_get_grid_info(self.h, _ptr(dims), _ptr(b)))
+        return dims, b
+
+    def value_ranges(self):
+        dims, _ = self.grid_info()
+        out = np.empty((int(np.prod(dims)), 2), np.float32)
+        check(lib().exa_sampler_get_value_ranges(self.h, _ptr(out)))
+        return out
+
+    def max_opacities(self):
+        dims, _ = self.grid_info()
+        out = np.empty(int(np.prod(dims)), np.float32)
+        check(lib().exa_sampler_get_max_opacities(self.h, _ptr(out)))
+        return out
+
+    def gridlet_value_ranges(self, num_gridlets):
+        out = np.empty((num_gridlets, 2), np.float32)
+        check(lib().exa_sampler_get_gridlet_value_ranges(self.h, _ptr(out)))
+        return out
+
+    def prim_max_opacities(self, num_gridlets, num_elements):
+        g = np.empty(max(num_gridlets, 1), np.float32)
+        e = np.empty(max(num_elements, 1), np.float32)
+        check(lib().exa_sampler_get_prim_max_opacities(self.h, _ptr(g), _ptr(e)))
+        return g[:num_gridlets], e[:num_elements]
+
+    def info(self):
+        i = np.zeros(8, np.int64)
+        check(lib().exa_sampler_get_info(self.h, _ptr(i)))
+        return dict(zip(("gridlet_nodes", "gridlet_depth", "gridlet_refs", "element_nodes", "element_depth",
+                         "element_refs", "model_bytes", "bvh_bytes"), (int(x) for x in i)))
+
+    def __del__(self):
+        if self._owned and getattr(self, "h", None):
+            lib().exa_sampler_destroy(self.h)
+            self.h = None
+
+
+class OWLRenderer:
+    """OWLRenderer: model -> sampler -> Woodcock path-traced frames."""
+
+    def __init__(self, model, num_mcs=(128, 128, 128), remap_from=None, remap_to=None):
+        self.h = C.c_void_p()
+        mcs = np.ascontiguousarray(num_mcs, dtype=np.int32)
+        rf = _f32(remap_from, 6) if remap_from is not None else None
+        rt = _f32(remap_to, 6) if remap_to is not None else None
+        check(lib().exa_renderer_create(model.h, _ptr(mcs), _ptr(rf), _ptr(rt), C.byref(self.h)))
+        self.sampler = ExaStitchSampler(lib().exa_renderer_get_sampler(self.h), owner=self)
+
+    # --- OWLRenderer.h:88-149 ---
+    def setCamera(self, org, dir_00, dir_du, dir_dv):
+        check(lib().exa_renderer_set_camera(self.h, _ptr(_f32(org, 3)), _ptr(_f32(dir_00, 3)),
+                                            _ptr(_f32(dir_du, 3)), _ptr(_f32(dir_dv, 3))))
+
+    def resize(self, w, h):
+        check(lib().exa_renderer_resize(self.h, int(w), int(h)))
+        self.size = (int(w), int(h))
+
+    def render(self, d_fb=None, stream=None):
+        ptr = C.c_void_p(d_fb.data_ptr()) if d_fb is not None else None
+        check(lib().exa_renderer_render(self.h, ptr, _stream(stream)))
+
+    def render_counted(self, d_fb=None, stream=None):
+        ptr = C.c_void_p(d_fb.data_ptr()) if d_fb is not None else None
+        c = np.zeros(10, np.uint64)
+        check(lib().exa_renderer_render_counted(self.h, ptr, _ptr(c), _stream(stream)))
+        names = ("gridlet", "tet", "pyr", "wedge", "hex", "miss", "cached", "dda_cells", "pixels", "tf")
+        return dict(zip(names, (int(x) for x in c)))
+
+    def accum(self):
+        w, h = self.size
+        out = np.empty((h, w, 4), np.float32)
+        check(lib().exa_renderer_get_accum(self.h, _ptr(out)))
+        return out
+
+    def setType(self, t):
+        check(lib().exa_renderer_set_type(self.h, int(t)))
+
+    def setColorMap(self, rgba):
+        cm = _f32(rgba).reshape(-1, 4)
+        check(lib().exa_renderer_set_color_map(self.h, _ptr(cm), len(cm)))
+
+    def setRange(self, lo, hi):
+        check(lib().exa_renderer_set_range(self.h, float(lo), float(hi)))
+
+    def setRelDomain(self, lo, hi):
+        check(lib().exa_renderer_set_rel_domain(self.h, float(lo), float(hi)))
+
+    def setOpacityScale(self, s):
+        check(lib().exa_renderer_set_opacity_scale(self.h, float(s)))
+
+    def setClipPlane(self, i, enabled, N, d):
+        check(lib().exa_renderer_set_clip_plane(self.h, int(i), int(bool(enabled)), _ptr(_f32(N, 3)), float(d)))
+
+    def setShadeMode(self, sm):
+        check(lib().exa_renderer_set_shade_mode(self.h, int(sm)))
+
+    def setLightSource(self, i, pos, intensity, on):
+        check(lib().exa_renderer_set_light_source(self.h, int(i), _ptr(_f32(pos, 3)), float(intensity),
+                                                  int(bool(on))))
+
+    def setDirectionalLight(self, i, direction, intensity, on):
+        check(lib().exa_renderer_set_directional_light(self.h, int(i), _ptr(_f32(direction, 3)), float(intensity),
+                                                       int(bool(on))))
+
+    def setLightSpaceTransform(self, xfm12):
+        check(lib().exa_renderer_set_light_space_transform(self.h, _ptr(_f32(xfm12, 12))))
+
+    def setSubImage(self, box, active):
+        check(lib().exa_renderer_set_sub_image(self.h, _ptr(_f32(box, 4)), int(bool(active))))
+
+    def setSubImageSelection(self, box, active):
+        check(lib().exa_renderer_set_sub_image_selection(self.h, _ptr(_f32(box, 4)), int(bool(active))))
+
+    def resetAccum(self):
+        check(lib().exa_renderer_reset_accum(self.h))
+
+    def setSpp(self, spp):
+        check(lib().exa_renderer_set_spp(self.h, int(spp)))
+
+    def setMaxBounces(self, n):
+        check(lib().exa_renderer_set_max_bounces(self.h, int(n)))
+
+    def setDebugCrosshair(self, on):
+        check(lib().exa_renderer_set_debug_crosshair(self.h, int(bool(on))))
+
+    @property
+    def modelBounds(self):
+        b = np.zeros(6, np.float32)
+        check(lib().exa_renderer_get_bounds(self.h, _ptr(b)))
+        return b
+
+    @property
+    def valueRange(self):
+        v = np.zeros(2, np.float32)
+        check(lib().exa_renderer_get_value_range(self.h, _ptr(v)))
+        return v
+
+    @property
+    def accumID(self):
+        return int(lib().exa_renderer_get_accum_id(self.h))
+
+    def state(self):
+        s = RenderState()
+        check(lib().exa_renderer_get_state(self.h, C.byref(s)))
+        return s
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().exa_renderer_destroy(self.h)
+            self.h = None
+
+
+def camera_look_at(frm, at, up, fovy, W, H):
+    org, d00, du, dv = (np.zeros(3, np.float32) for _ in range(4))
+    check(lib().exa_camera_look_at(_ptr(_f32(frm, 3)), _ptr(_f32(at, 3)), _ptr(_f32(up, 3)), float(fovy), int(W),
+                                   int(H), _ptr(org), _ptr(d00), _ptr(du), _ptr(dv)))
+    return org, d00, du, dv
+
+
+def tf_rasterize(points, N=128):
+    pts = _f32(points).reshape(-1, 5)
+    out = np.empty((N, 4), np.float32)
+    check(lib().exa_tf_rasterize(_ptr(pts), len(pts), int(N), _ptr(out)))
+    return out
+
+
+def xf_load(path):
+    os_ = C.c_float()
+    ad = np.zeros(2, np.float32)
+    rd = np.zeros(2, np.float32)
+    n = C.c_int()
+    check(lib().exa_xf_load(path.encode(), C.byref(os_), _ptr(ad), _ptr(rd), C.byref(n), None, 0))
+    cm = np.empty((n.value, 4), np.float32)
+    check(lib().exa_xf_load(path.encode(), None, None, None, None, _ptr(cm), n.value))
+    return {"opacityScale": os_.value, "absDomain": ad, "relDomain": rd, "colorMap": cm}
