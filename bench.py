@@ -1,0 +1,245 @@
+"""Benchmark: path-traced samples/s at 1920x1080 on one MI355X.
+
+Workload (the medium GPU configuration): 3-level AMR, refinement ratio 2,
+base 128^3 cells of width 4 over [0,512]^3, refined where 6-octave fractal
+value noise exceeds 0.55 and 0.7 (about 31M leaf cells), scalar = noise x
+radial falloff; dual mesh with gridlets of <= 8^3 cells; 64^3 majorant grid;
+1920x1080, 1 spp per frame, Woodcock-tracked path tracing capped at 4
+bounces with one directional light.  One step = one frame.
+
+Prints ONE JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "path-traced samples/sec @1080p on 1 GPU; neural vs exact sampler at fixed RMSE"
+HBM_PEAK_GBS = 8000.0
+
+# algorithmic bytes per unit of work (leaf records only, traversal nodes are
+# overhead): gridlet hit = 32 B record + 8 corners x 4 B; element hit = 32 B
+# index row + 16 B per vertex; a cached gridlet hit re-reads the same 64 B;
+# 4 B per macrocell visited; 36 B per pixel (accum read + write + fb write)
+BYTES = {"gridlet": 64, "tet": 96, "pyr": 112, "wedge": 128, "hex": 160, "cached": 64, "dda_cells": 4,
+         "pixels": 36}
+
+TF5 = np.array([
+    [0.00, 0.0, 0.0, 0.3, 0.00],
+    [0.20, 0.1, 0.4, 0.9, 0.05],
+    [0.45, 0.2, 0.9, 0.5, 0.30],
+    [0.70, 0.9, 0.8, 0.1, 0.60],
+    [1.00, 1.0, 0.2, 0.1, 1.00],
+], np.float32)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_scene(args):
+    from owlexastitcher_amd import AmrScene, DualMesh, ExaStitchModel
+    t = time.time()
+    amr = AmrScene.noise(args.base_res, 2, 0.55, 0.7, 6, 2.0, args.seed)
+    log(f"[bench] AMR: {len(amr)} leaf cells ({time.time() - t:.1f}s)")
+    t = time.time()
+    dm = DualMesh(amr)
+    counts = dm.counts()
+    log(f"[bench] dual mesh: {counts} ({time.time() - t:.1f}s)")
+    model = ExaStitchModel.from_dual_mesh(dm)
+    return amr, dm, model, counts
+
+
+def setup_renderer(model, args):
+    from owlexastitcher_amd import OWLRenderer, camera_look_at, tf_rasterize
+    t = time.time()
+    r = OWLRenderer(model, (args.mcs, args.mcs, args.mcs))
+    log(f"[bench] renderer + sampler build: {time.time() - t:.1f}s, {r.sampler.info()}")
+    b = r.modelBounds
+    center = 0.5 * (b[:3] + b[3:])
+    span = b[3:] - b[:3]
+    org, d00, du, dv = camera_look_at(center + np.array([-0.3, 0.7, 1.0]) * span, center, (0, 1, 0), 70.0,
+                                      args.width, args.height)
+    r.resize(args.width, args.height)
+    r.setCamera(org, d00, du, dv)
+    rgba = tf_rasterize(TF5, 128)
+    r.setColorMap(rgba)
+    vr = r.valueRange
+    r.setRange(vr[0], vr[1])
+    r.setRelDomain(0.0, 100.0)
+    r.setOpacityScale(100.0)
+    r.setType(args.integrator)
+    r.setMaxBounces(args.bounces)
+    r.setSpp(args.spp)
+    r.setDirectionalLight(0, (0.3, 1.0, 0.5), 2.0, True)
+    if args.accel == "bvh":
+        from owlexastitcher_amd import ACCEL_BVH
+        t = time.time()
+        r.setAccel(ACCEL_BVH)
+        log(f"[bench] BVH accelerator built ({time.time() - t:.1f}s)")
+    return r, rgba
+
+
+def algorithmic_bytes(c):
+    return sum(BYTES[k] * c[k] for k in BYTES)
+
+
+def cpu_baseline(r, rgba, dm, args, target_s):
+    """The oracle (single-precision restatement of the reference's sampling
+    and integration code) on this host's cores, on a band of rows of the
+    same frame."""
+    from oracle import oracle as ora
+    st = r.state()
+    idx, verts, gl, gs = dm.arrays()
+    ng, ne = len(gl), len(idx)
+    gmo, emo = r.sampler.prim_max_opacities(ng, ne)
+    om = ora.Model(verts, idx, gl, gs, gmo, emo)
+    t = time.time()
+    om.bins(64)
+    log(f"[bench] cpu baseline: oracle bins built ({time.time() - t:.1f}s)")
+    maj = r.sampler.max_opacities()
+    threads = os.cpu_count() or 1
+    rows = 8
+    elapsed = 0.0
+    done_rows = 0
+    y0 = st.H // 2 - 4 * rows
+    while elapsed < target_s and done_rows < st.H:
+        st.H = st.H  # full-frame parameters; only rows [y0, y0+rows) are rendered
+        p = ora.render_params_from_state(st, maj, rgba)
+        accum = np.zeros((st.H, st.W, 4), np.float32)
+        t = time.time()
+        _render_rows(ora, p, om, accum, y0 + done_rows, min(st.H, y0 + done_rows + rows), threads)
+        elapsed += time.time() - t
+        done_rows += rows
+        rows = min(st.H - done_rows, rows * 2)
+        if rows <= 0:
+            break
+    samples = done_rows * st.W * st.spp
+    return {"value": samples / elapsed, "unit": "samples/s", "cores": threads, "kind": "port",
+            "sample": f"{done_rows} rows x {st.W} px x {st.spp} spp of the same 1920x1080 path-traced frame, "
+                      f"oracle f32 restatement on {threads} host threads, {elapsed:.1f}s"}
+
+
+def _render_rows(ora, p, om, accum, y0, y1, threads):
+    import ctypes as C
+    from concurrent.futures import ThreadPoolExecutor
+    fb = np.zeros((p.H, p.W), np.uint32)
+    bins = om.bins()
+
+    def work(y):
+        ora.lib().ora_render_f32(C.byref(p), om.ref, bins, ora.ptr(accum), ora.ptr(fb), y, y + 1)
+
+    with ThreadPoolExecutor(threads) as ex:
+        list(ex.map(work, range(y0, y1)))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=32)
+    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--spp", type=int, default=1)
+    ap.add_argument("--bounces", type=int, default=4)
+    ap.add_argument("--integrator", type=int, default=0, help="0 path tracer, 1 single scatter")
+    ap.add_argument("--base-res", type=int, default=128)
+    ap.add_argument("--mcs", type=int, default=64)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--accel", choices=["tables", "bvh"], default="tables")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build measures one GPU only (the renderer shards in image space "
+              "across GPUs by design, but no multi-GPU path is built); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from owlexastitcher_amd import lib
+    lib()
+    dev = torch.device("cuda:0")
+    amr, dm, model, counts = build_scene(args)
+    r, rgba = setup_renderer(model, args)
+    stream = torch.cuda.current_stream()
+    fb = torch.zeros((args.height, args.width), dtype=torch.int32, device=dev)
+
+    log(f"[bench] warmup {args.warmup} frames")
+    for _ in range(args.warmup):
+        r.render(fb, stream)
+    torch.cuda.synchronize()
+    first_timed = r.accumID
+    log(f"[bench] timing {args.steps} frames")
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev[0].record(stream)
+    for i in range(args.steps):
+        r.render(fb, stream)
+        ev[i + 1].record(stream)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    wall = t1 - t0
+    frame_ms = [ev[i].elapsed_time(ev[i + 1]) for i in range(args.steps)]
+    kernel_ms = float(np.mean(frame_ms))
+    log(f"[bench] frames: mean {kernel_ms:.2f} ms, min {min(frame_ms):.2f}, max {max(frame_ms):.2f}; wall {wall:.2f}s")
+
+    # instrumented frame with the seeds of the first timed frame
+    r.accumID = first_timed
+    cnt = r.render_counted(fb, stream)
+    alg = algorithmic_bytes(cnt)
+    log(f"[bench] counters (frame accumID={first_timed}): {cnt}; algorithmic bytes {alg / 1e6:.1f} MB")
+    achieved = alg / (kernel_ms * 1e-3) / 1e9
+
+    samples_per_frame = args.width * args.height * args.spp
+    value = samples_per_frame * args.steps / wall
+    queries = sum(cnt[k] for k in ("gridlet", "tet", "pyr", "wedge", "hex", "miss", "cached"))
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        try:
+            cpu = cpu_baseline(r, rgba, dm, args, args.cpu_seconds)
+            log(f"[bench] cpu baseline: {cpu}")
+        except Exception as e:  # the baseline is reported, not required
+            log(f"[bench] cpu baseline failed: {e!r}")
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"3-level fractal-noise AMR ({len(amr)} leaf cells, base {args.base_res}^3 x width 4), "
+                         f"dual mesh {counts['elements']} stitching elements + {counts['gridlets']} gridlets, "
+                         f"{args.mcs}^3 majorant grid, {args.width}x{args.height}, {args.spp} spp/frame, "
+                         + ("path tracer <= %d bounces" % args.bounces if args.integrator == 0
+                            else "single scatter") + ", 1 directional light"),
+            "accel": args.accel,
+            "frame_ms": kernel_ms,
+            "volume_samples_per_s": queries / (kernel_ms * 1e-3),
+            "counters": cnt,
+        },
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "kRender<false>"},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -127,8 +127,17 @@ int exa_sampler_get_value_ranges(const exa_sampler *s, float *out);          /* 
 int exa_sampler_get_max_opacities(const exa_sampler *s, float *out);         /* host, 1 per macrocell */
 int exa_sampler_get_gridlet_value_ranges(const exa_sampler *s, float *out);  /* host, 2 per gridlet */
 int exa_sampler_get_prim_max_opacities(const exa_sampler *s, float *gridletOut, float *elementOut);
+/* Point-location accelerator (replaces the OptiX sampling BVH,
+   ExaStitchSampler.cpp:71-323): EXA_ACCEL_TABLES (default) = per-level
+   macrocell lists of gridlets + uniform element bins; EXA_ACCEL_BVH = binary
+   BVHs.  Both return the same samples.  Selecting an accelerator that was not
+   built fails with EXA_ERR_STATE after build. */
+#define EXA_ACCEL_TABLES 0
+#define EXA_ACCEL_BVH 1
+int exa_sampler_set_accel(exa_sampler *s, int accel);
+int exa_sampler_get_accel(const exa_sampler *s);
 /* info[8]: gridlet BVH nodes, depth, refs; element BVH nodes, depth, refs;
-   device bytes of the model; device bytes of the BVHs */
+   device bytes of the model; device bytes of the accelerators */
 int exa_sampler_get_info(const exa_sampler *s, int64_t info[8]);
 void exa_sampler_destroy(exa_sampler *s);
 
@@ -174,7 +183,13 @@ int exa_renderer_set_debug_crosshair(exa_renderer *r, int on);
 int exa_renderer_get_bounds(const exa_renderer *r, float modelBounds[6]);
 int exa_renderer_get_value_range(const exa_renderer *r, float valueRange[2]);
 int exa_renderer_get_accum_id(const exa_renderer *r);
+/* OWLRenderer::accumID is a public field (OWLRenderer.h:163); the next
+   render() seeds its RNG with (pixelID, accumID) */
+int exa_renderer_set_accum_id(exa_renderer *r, int accumID);
 const exa_sampler *exa_renderer_get_sampler(const exa_renderer *r);
+/* selects the sampler's accelerator (building it from the renderer's model
+   copy when needed) */
+int exa_renderer_set_accel(exa_renderer *r, int accel);
 
 /* Snapshot of the launch parameters of the next render() call (for checking
    against the CPU oracle). */

@@ -99,7 +99,8 @@ def lib():
 
 
 def ptr(a):
-    return None if a is None else C.c_void_p(a.ctypes.data)
+    # data_as keeps a reference to the array for the lifetime of the pointer
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
 
 
 class Model:

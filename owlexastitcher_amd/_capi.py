@@ -102,6 +102,8 @@ _SIGS = {
     "exa_sampler_get_gridlet_value_ranges": (C.c_int, [P, P]),
     "exa_sampler_get_prim_max_opacities": (C.c_int, [P, P, P]),
     "exa_sampler_get_info": (C.c_int, [P, P]),
+    "exa_sampler_set_accel": (C.c_int, [P, C.c_int]),
+    "exa_sampler_get_accel": (C.c_int, [P]),
     "exa_sampler_destroy": (None, [P]),
     "exa_renderer_create": (C.c_int, [P, P, P, P, P]),
     "exa_renderer_set_camera": (C.c_int, [P, P, P, P, P]),
@@ -128,7 +130,9 @@ _SIGS = {
     "exa_renderer_get_bounds": (C.c_int, [P, P]),
     "exa_renderer_get_value_range": (C.c_int, [P, P]),
     "exa_renderer_get_accum_id": (C.c_int, [P]),
+    "exa_renderer_set_accum_id": (C.c_int, [P, C.c_int]),
     "exa_renderer_get_sampler": (P, [P]),
+    "exa_renderer_set_accel": (C.c_int, [P, C.c_int]),
     "exa_renderer_get_state": (C.c_int, [P, P]),
     "exa_renderer_destroy": (None, [P]),
     "exa_camera_look_at": (C.c_int, [P, P, P, F, C.c_int, C.c_int, P, P, P, P]),

@@ -150,10 +150,16 @@ struct exa_sampler {
   DeviceBuffer<Gridlet> gridlets;
   DeviceBuffer<float> gridletScalars;
   int64_t numElements = 0, numGridlets = 0;
-  // point-location BVHs
+  // point-location accelerators: direct tables (default) and BVHs
+  int accel = kAccelTables;
+  bool wantBvh = false, haveBvh = false, haveTables = false;
   DeviceBuffer<BvhNode> gNodes, eNodes;
   DeviceBuffer<int32_t> gRefs, eRefs;
   int gDepth = 0, eDepth = 0;
+  LocateTables tabMeta;  // sizes/offsets only (vectors are released after upload)
+  DeviceBuffer<uint32_t> tGStart, tEStart;
+  DeviceBuffer<int32_t> tGList, tEList;
+  DeviceBuffer<float4> tEBoxes;
   // gridlet value ranges + per-primitive max opacities (culling)
   DeviceBuffer<float2> gridletRanges;
   DeviceBuffer<float> gridletMaxOp, elemMaxOp;
@@ -181,7 +187,79 @@ struct exa_sampler {
     v.eNodes = eNodes.ptr;
     v.eRefs = eRefs.ptr;
     v.numElements = (int32_t)numElements;
+    v.gStart = tGStart.ptr;
+    v.gList = tGList.ptr;
+    v.numLevels = tabMeta.numLevels;
+    for (int L = 0; L < LocateTables::kMaxLevels; ++L) {
+      for (int k = 0; k < 3; ++k) v.levelDims[L][k] = tabMeta.levelDims[L][k];
+      v.levelOffset[L] = (int32_t)tabMeta.levelOffset[L];
+    }
+    v.eStart = tEStart.ptr;
+    v.eList = tEList.ptr;
+    v.eBoxes = tEBoxes.ptr;
+    for (int k = 0; k < 3; ++k) {
+      v.binLo[k] = tabMeta.binLo[k];
+      v.binDims[k] = tabMeta.binDims[k];
+    }
+    v.binInv = tabMeta.binInv;
     return v;
+  }
+
+  void buildBvhs(const HostModel &m, hipStream_t s) {
+    std::vector<float> boxes;
+    Bvh bvh;
+    gridletBoxes(m.gridlets, boxes);
+    buildBvh(boxes.data(), (int64_t)m.gridlets.size(), 4, bvh);
+    gNodes.upload(bvh.nodes.data(), bvh.nodes.size(), s);
+    gRefs.upload(bvh.primRefs.data(), bvh.primRefs.size(), s);
+    gDepth = bvh.depth;
+    HIP_CHECK(hipStreamSynchronize(s));
+    elementBoxes(m.indices, m.vertices, boxes);
+    buildBvh(boxes.data(), m.numElements(), 4, bvh);
+    eNodes.upload(bvh.nodes.data(), bvh.nodes.size(), s);
+    eRefs.upload(bvh.primRefs.data(), bvh.primRefs.size(), s);
+    eDepth = bvh.depth;
+    HIP_CHECK(hipStreamSynchronize(s));
+    haveBvh = true;
+  }
+
+  void buildTables(const HostModel &m, hipStream_t s) {
+    LocateTables t;
+    buildLocateTables(m, t);
+    tGStart.upload(t.gStart.data(), t.gStart.size(), s);
+    tGList.upload(t.gList.data(), t.gList.size(), s);
+    tEStart.upload(t.eStart.data(), t.eStart.size(), s);
+    tEList.upload(t.eList.data(), t.eList.size(), s);
+    tEBoxes.upload((const float4 *)t.eBoxes.data(), t.eBoxes.size() / 4, s);
+    HIP_CHECK(hipStreamSynchronize(s));
+    tabMeta = LocateTables();
+    tabMeta.numLevels = t.numLevels;
+    for (int L = 0; L < LocateTables::kMaxLevels; ++L) {
+      for (int k = 0; k < 3; ++k) tabMeta.levelDims[L][k] = t.levelDims[L][k];
+      tabMeta.levelOffset[L] = t.levelOffset[L];
+    }
+    for (int k = 0; k < 3; ++k) {
+      tabMeta.binLo[k] = t.binLo[k];
+      tabMeta.binDims[k] = t.binDims[k];
+    }
+    tabMeta.binInv = t.binInv;
+    haveTables = true;
+  }
+
+  // selects the accelerator; builds it from m when missing
+  void setAccel(int a, const HostModel *m, hipStream_t s) {
+    if (a != kAccelTables && a != kAccelBvh) throw Status(EXA_ERR_INVALID, "unknown accelerator");
+    if (built) {
+      if (a == kAccelBvh && !haveBvh) {
+        if (!m) throw Status(EXA_ERR_STATE, "BVH not built (select it before build)");
+        buildBvhs(*m, s);
+      }
+      if (a == kAccelTables && !haveTables) {
+        if (!m) throw Status(EXA_ERR_STATE, "tables not built (select them before build)");
+        buildTables(*m, s);
+      }
+    }
+    accel = a;
   }
   GridView gridView() const {
     GridView g;
@@ -208,20 +286,15 @@ struct exa_sampler {
     indices.upload(m.indices.data(), m.indices.size(), s);
     gridlets.upload(m.gridlets.data(), m.gridlets.size(), s);
     gridletScalars.upload(m.gridletScalars.data(), m.gridletScalars.size(), s);
-    // BVHs (the OptiX BLASes over GridletGeomBounds / StitchGeomBounds)
-    std::vector<float> boxes;
-    Bvh bvh;
-    gridletBoxes(m.gridlets, boxes);
-    buildBvh(boxes.data(), numGridlets, 4, bvh);
-    gNodes.upload(bvh.nodes.data(), bvh.nodes.size(), s);
-    gRefs.upload(bvh.primRefs.data(), bvh.primRefs.size(), s);
-    gDepth = bvh.depth;
-    HIP_CHECK(hipStreamSynchronize(s));
-    elementBoxes(m.indices, m.vertices, boxes);
-    buildBvh(boxes.data(), numElements, 4, bvh);
-    eNodes.upload(bvh.nodes.data(), bvh.nodes.size(), s);
-    eRefs.upload(bvh.primRefs.data(), bvh.primRefs.size(), s);
-    eDepth = bvh.depth;
+    // point-location accelerators (replace the OptiX BLASes over
+    // GridletGeomBounds / StitchGeomBounds and the TLAS)
+    haveBvh = haveTables = false;
+    bool tablesOk = true;
+    for (const Gridlet &g : m.gridlets)
+      if (g.lower[0] < 0 || g.lower[1] < 0 || g.lower[2] < 0) tablesOk = false;
+    if (accel == kAccelTables && !tablesOk) accel = kAccelBvh;
+    if (accel == kAccelTables) buildTables(m, s);
+    if (accel == kAccelBvh || wantBvh) buildBvhs(m, s);
     // computeGridletValueRanges (ExaStitchSampler.cu:229-277)
     gridletRanges.alloc(numGridlets);
     gridletMaxOp.alloc(numGridlets);
@@ -467,7 +540,7 @@ int exa_sampler_locate(const exa_sampler *s, const float *d_pos, int64_t n, exa_
   return guarded([&] {
     require(s && (n == 0 || (d_pos && d_out)) && n >= 0, "invalid arguments");
     if (!s->built) throw Status(EXA_ERR_STATE, "sampler not built");
-    launchLocate(s->view(), d_pos, n, (Sample *)d_out, nullptr, (hipStream_t)stream);
+    launchLocate(s->view(), s->accel, d_pos, n, (Sample *)d_out, nullptr, (hipStream_t)stream);
     HIP_CHECK(hipGetLastError());
   });
 }
@@ -478,7 +551,7 @@ int exa_sampler_locate_counted(const exa_sampler *s, const float *d_pos, int64_t
     if (!s->built) throw Status(EXA_ERR_STATE, "sampler not built");
     hipStream_t st = (hipStream_t)stream;
     HIP_CHECK(hipMemsetAsync(s->counters.ptr, 0, kNumCounters * sizeof(unsigned long long), st));
-    launchLocate(s->view(), d_pos, n, (Sample *)d_out, s->counters.ptr, st);
+    launchLocate(s->view(), s->accel, d_pos, n, (Sample *)d_out, s->counters.ptr, st);
     HIP_CHECK(hipGetLastError());
     unsigned long long h[kNumCounters];
     HIP_CHECK(hipMemcpyAsync(h, s->counters.ptr, sizeof(h), hipMemcpyDeviceToHost, st));
@@ -539,9 +612,23 @@ int exa_sampler_get_info(const exa_sampler *s, int64_t info[8]) {
     info[4] = s->eDepth;
     info[5] = (int64_t)s->eRefs.count;
     info[6] = (int64_t)(s->vertices.bytes() + s->indices.bytes() + s->gridlets.bytes() + s->gridletScalars.bytes());
-    info[7] = (int64_t)(s->gNodes.bytes() + s->eNodes.bytes() + s->gRefs.bytes() + s->eRefs.bytes());
+    info[7] = (int64_t)(s->gNodes.bytes() + s->eNodes.bytes() + s->gRefs.bytes() + s->eRefs.bytes() +
+                        s->tGStart.bytes() + s->tGList.bytes() + s->tEStart.bytes() + s->tEList.bytes() +
+                        s->tEBoxes.bytes());
   });
 }
+int exa_sampler_set_accel(exa_sampler *s, int accel) {
+  return guarded([&] {
+    require(s, "sampler is NULL");
+    if (!s->built) {
+      if (accel != kAccelTables && accel != kAccelBvh) throw Status(EXA_ERR_INVALID, "unknown accelerator");
+      s->accel = accel;
+      return;
+    }
+    s->setAccel(accel, nullptr, 0);
+  });
+}
+int exa_sampler_get_accel(const exa_sampler *s) { return s ? s->accel : EXA_ERR_INVALID; }
 void exa_sampler_destroy(exa_sampler *s) { delete s; }
 
 // ---- renderer --------------------------------------------------------------
@@ -626,7 +713,7 @@ static void doRender(exa_renderer *r, uint32_t *fb, unsigned long long *counters
   P.spp = r->spp < 1 ? 1 : r->spp;
   P.tf = r->tf.ptr;
   P.tfN = (int32_t)r->tf.count;
-  launchRender(P, r->sampler.view(), r->sampler.gridView(), counters, s);
+  launchRender(P, r->sampler.view(), r->sampler.accel, r->sampler.gridView(), counters, s);
   HIP_CHECK(hipGetLastError());
   r->accumID++;
 }
@@ -790,7 +877,18 @@ int exa_renderer_get_value_range(const exa_renderer *r, float v[2]) {
   return EXA_OK;
 }
 int exa_renderer_get_accum_id(const exa_renderer *r) { return r ? r->accumID : EXA_ERR_INVALID; }
+int exa_renderer_set_accum_id(exa_renderer *r, int accumID) {
+  if (!r || accumID < 0) return EXA_ERR_INVALID;
+  r->accumID = accumID;
+  return EXA_OK;
+}
 const exa_sampler *exa_renderer_get_sampler(const exa_renderer *r) { return r ? &r->sampler : nullptr; }
+int exa_renderer_set_accel(exa_renderer *r, int accel) {
+  return guarded([&] {
+    require(r, "renderer is NULL");
+    r->sampler.setAccel(accel, &r->model, 0);
+  });
+}
 int exa_renderer_get_state(const exa_renderer *r, exa_render_state *o) {
   return guarded([&] {
     require(r && o, "invalid arguments");

@@ -147,9 +147,11 @@ struct Exa {
     f = floorf(f / (float)(1 << level));
     return (int)(f * (float)(1 << level));
   }
-  // makeDual.cpp:172-195
-  int64_t find(const float where[3]) const {
-    for (int level = minLevel; level <= maxLevel; level++) {
+  // makeDual.cpp:172-195 Exa::find, restricted to levels >= fromLevel: a
+  // corner found on a finer level makes doCell skip the octant just like a
+  // missing corner, so finer levels need not be probed.
+  int64_t find(const float where[3], int fromLevel) const {
+    for (int level = fromLevel; level <= maxLevel; level++) {
       const int x = lowerOnLevel(where[0], level), y = lowerOnLevel(where[1], level),
                 z = lowerOnLevel(where[2], level);
       if (x < 0 || y < 0 || z < 0 || x >= kCoordLimit || y >= kCoordLimit || z >= kCoordLimit)
@@ -172,7 +174,7 @@ inline bool cellLess(const SortedCell &a, const SortedCell &b) {
 }
 
 // makeDual.cpp:596-872 doCell, with early outs that cannot change the result:
-// an octant is dropped as soon as a corner is missing, coarser than the cell,
+// an octant is dropped as soon as a corner is missing, finer than the cell,
 // or a same-level corner precedes the cell in sorted order (each of these
 // makes the reference skip the octant).
 void doCell(const Exa &exa, int64_t ci, ChunkOut &out) {
@@ -194,10 +196,9 @@ void doCell(const Exa &exa, int64_t ci, ChunkOut &out) {
               const float c[3] = {(float)(cell.pos[0] + dx * ix * cw) + 0.5f * (float)cw,
                                   (float)(cell.pos[1] + dy * iy * cw) + 0.5f * (float)cw,
                                   (float)(cell.pos[2] + dz * iz * cw) + 0.5f * (float)cw};
-              const int64_t f = exa.find(c);
+              const int64_t f = exa.find(c, cell.level);
               if (f < 0) { skip = true; break; }
               const SortedCell &fc = exa.cells[f];
-              if (fc.level < cell.level) { skip = true; break; }
               if (fc.level == cell.level && f < ci) { skip = true; break; }
               corner[iz][iy][ix] = f;
               minLevel = std::min(minLevel, fc.level);

@@ -95,6 +95,34 @@ constexpr int kBvhMaxDepth = 32;
 void buildBvh(const float *boxes, int64_t n, int leafSize, Bvh &out);
 
 void gridletBoxes(const std::vector<Gridlet> &g, std::vector<float> &boxes);
+
+// ---------------------------------------------------------------------------
+// direct point-location tables (the default accelerator)
+// ---------------------------------------------------------------------------
+// Gridlets: per refinement level L, a grid of 8^3-cell macrocells; the list
+// of macrocell m holds every level-L gridlet whose box reaches into m (box
+// [(lower+.5)2^L, (lower+dims+.5)2^L] <=> t = p/2^L - .5 in [lower,
+// lower+dims]), ascending.  A point only needs the list of floor(t/8) per
+// level.
+// Elements: a uniform bin grid over the element AABBs; the list of a bin
+// holds every element whose AABB overlaps it, ascending.  Bin indices are
+// computed with the same float expression on host and device, so every
+// element whose AABB contains a point is in that point's bin.
+struct LocateTables {
+  static constexpr int kMaxLevels = 16;
+  int32_t numLevels = 0;                 // levels 0..numLevels-1 (may be empty)
+  int32_t levelDims[kMaxLevels][3] = {};  // macrocells per axis
+  int64_t levelOffset[kMaxLevels + 1] = {};
+  std::vector<uint32_t> gStart;  // per macrocell (all levels concatenated) + 1
+  std::vector<int32_t> gList;
+  float binLo[3] = {0, 0, 0};
+  float binInv = 1.f;  // 1 / bin size (a power of two)
+  int32_t binDims[3] = {0, 0, 0};
+  std::vector<uint32_t> eStart;  // per bin + 1
+  std::vector<int32_t> eList;
+  std::vector<float> eBoxes;  // 8 per element: lo.xyz, 0, hi.xyz, 0
+};
+void buildLocateTables(const HostModel &m, LocateTables &t);
 void elementBoxes(const std::vector<int32_t> &indices, const std::vector<float> &vertices,
                   std::vector<float> &boxes);
 

@@ -24,7 +24,21 @@ struct SamplerView {
   const BvhNode *eNodes;
   const int32_t *eRefs;
   int32_t numElements;
+  // direct tables (LocateTables)
+  const uint32_t *gStart;
+  const int32_t *gList;
+  int32_t numLevels;
+  int32_t levelDims[LocateTables::kMaxLevels][3];
+  int32_t levelOffset[LocateTables::kMaxLevels];
+  const uint32_t *eStart;
+  const int32_t *eList;
+  const float4 *eBoxes;
+  float binLo[3];
+  float binInv;
+  int32_t binDims[3];
 };
+
+enum Accel { kAccelTables = 0, kAccelBvh = 1 };
 
 // GridTraversable (Grid.cuh:21-24) plus the majorant buffer.
 struct GridView {
@@ -62,7 +76,7 @@ void launchMapMaxOpacity(const float2 *ranges, int64_t n, const float4 *cmap, in
 void launchElementMaxOpacity(const float4 *vertices, const int32_t *indices, int64_t n,
                              const float4 *cmap, int numColors, float x0, float x1, float *out,
                              hipStream_t s);
-void launchLocate(const SamplerView &sv, const float *pos, int64_t n, Sample *out,
+void launchLocate(const SamplerView &sv, int accel, const float *pos, int64_t n, Sample *out,
                   unsigned long long *counters, hipStream_t s);
 void launchTestSample(const SamplerView &sv, const float *pos, const int32_t *primIDs, int64_t n,
                       Sample *out, hipStream_t s);
@@ -104,7 +118,7 @@ struct RenderParams {
   uint32_t *fb;
 };
 
-void launchRender(const RenderParams &p, const SamplerView &sv, const GridView &gv,
+void launchRender(const RenderParams &p, const SamplerView &sv, int accel, const GridView &gv,
                   unsigned long long *counters, hipStream_t s);
 
 }  // namespace exa

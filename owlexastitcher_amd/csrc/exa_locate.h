@@ -51,7 +51,9 @@ EXA_DEV void bvhPointQuery(const BvhNode *__restrict__ nodes, const vec3f p, int
 
 // Element test with the AABB filter of StitchGeomBounds
 // (programs/ExaStitchSampler.cu:73-94); returns the element type (0 tet,
-// 1 pyramid, 2 wedge, 3 hex) on a hit, -1 otherwise.
+// 1 pyramid, 2 wedge, 3 hex) on a hit, -1 otherwise.  kCheckBox = false when
+// the caller has already tested the stored AABB (same values).
+template <bool kCheckBox = true>
 EXA_DEV int testElement(float &value, vec3f P, const int32_t *__restrict__ idx8,
                         const float4 *__restrict__ vertices) {
   const int4 i0 = *(const int4 *)idx8;
@@ -65,16 +67,18 @@ EXA_DEV int testElement(float &value, vec3f P, const int32_t *__restrict__ idx8,
   if (nv > 4) v[4] = vertices[i1.x];
   if (nv > 5) v[5] = vertices[i1.y];
   if (nv > 6) { v[6] = vertices[i1.z]; v[7] = vertices[i1.w]; }
-  float lo[3] = {v[0].x, v[0].y, v[0].z}, hi[3] = {v[0].x, v[0].y, v[0].z};
+  if (kCheckBox) {
+    float lo[3] = {v[0].x, v[0].y, v[0].z}, hi[3] = {v[0].x, v[0].y, v[0].z};
 #pragma unroll
-  for (int i = 1; i < 8; ++i)
-    if (i < nv) {
-      lo[0] = fminf(lo[0], v[i].x); hi[0] = fmaxf(hi[0], v[i].x);
-      lo[1] = fminf(lo[1], v[i].y); hi[1] = fmaxf(hi[1], v[i].y);
-      lo[2] = fminf(lo[2], v[i].z); hi[2] = fmaxf(hi[2], v[i].z);
-    }
-  if (!(P.x >= lo[0] && P.y >= lo[1] && P.z >= lo[2] && P.x <= hi[0] && P.y <= hi[1] && P.z <= hi[2]))
-    return -1;
+    for (int i = 1; i < 8; ++i)
+      if (i < nv) {
+        lo[0] = fminf(lo[0], v[i].x); hi[0] = fmaxf(hi[0], v[i].x);
+        lo[1] = fminf(lo[1], v[i].y); hi[1] = fmaxf(hi[1], v[i].y);
+        lo[2] = fminf(lo[2], v[i].z); hi[2] = fmaxf(hi[2], v[i].z);
+      }
+    if (!(P.x >= lo[0] && P.y >= lo[1] && P.z >= lo[2] && P.x <= hi[0] && P.y <= hi[1] && P.z <= hi[2]))
+      return -1;
+  }
   bool hit;
   if (nv == 4) hit = intersectTet(value, P, v[0], v[1], v[2], v[3]);
   else if (nv == 5) hit = intersectNewton<5>(value, P, v);
@@ -85,9 +89,78 @@ EXA_DEV int testElement(float &value, vec3f P, const int32_t *__restrict__ idx8,
   return nv == 4 ? 0 : (nv == 5 ? 1 : (nv == 6 ? 2 : 3));
 }
 
-// Returns the sample; kind receives -1 miss, 0 gridlet, 1..4 tet/pyr/wedge/hex.
-EXA_DEV Sample locatePoint(const SamplerView &sv, const vec3f p, int32_t *stack, const int stride,
-                           int &kind) {
+// Direct-table point location (LocateTables): per level, the gridlet list of
+// macrocell floor((p/2^L - .5)/8); then the element list of p's bin.
+EXA_DEV Sample locateTables(const SamplerView &sv, const vec3f p, int &kind) {
+  Sample s{-1, -1, 0.f};
+  kind = -1;
+  int32_t best = 0x7fffffff;
+  for (int L = 0; L < sv.numLevels; ++L) {
+    const int32_t *D = sv.levelDims[L];
+    const float inv = 1.f / (float)(1 << L);
+    const float mxf = floorf((p.x * inv - 0.5f) * 0.125f);
+    const float myf = floorf((p.y * inv - 0.5f) * 0.125f);
+    const float mzf = floorf((p.z * inv - 0.5f) * 0.125f);
+    if (!(mxf >= 0.f && myf >= 0.f && mzf >= 0.f && mxf < (float)D[0] && myf < (float)D[1] &&
+          mzf < (float)D[2]))
+      continue;
+    const int32_t idx = sv.levelOffset[L] + ((int32_t)mzf * D[1] + (int32_t)myf) * D[0] + (int32_t)mxf;
+    const uint32_t k1 = sv.gStart[idx + 1];
+    for (uint32_t k = sv.gStart[idx]; k < k1; ++k) {
+      const int32_t g = sv.gList[k];
+      if (g >= best) break;
+      if (sv.gridletMaxOp && sv.gridletMaxOp[g] == 0.f) continue;
+      float v;
+      int cid;
+      const int r = intersectGridlet(v, cid, p, sv.gridlets[g], sv.gridletScalars);
+      if (r) {
+        best = g;
+        s.primID = g;
+        s.cellID = cid;
+        s.value = v;
+        if (r == 2) {
+          kind = 0;
+          return s;
+        }
+        break;
+      }
+    }
+  }
+  if (s.primID >= 0) {
+    kind = 0;
+    return s;
+  }
+  if (sv.numElements > 0) {
+    int32_t b[3];
+    const float q[3] = {p.x, p.y, p.z};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const float f = floorf((q[k] - sv.binLo[k]) * sv.binInv);
+      b[k] = f != f ? 0 : (f < 0.f ? 0 : (f > (float)(sv.binDims[k] - 1) ? sv.binDims[k] - 1 : (int32_t)f));
+    }
+    const int64_t bin = ((int64_t)b[2] * sv.binDims[1] + b[1]) * sv.binDims[0] + b[0];
+    const uint32_t k1 = sv.eStart[bin + 1];
+    for (uint32_t k = sv.eStart[bin]; k < k1; ++k) {
+      const int32_t e = sv.eList[k];
+      if (sv.elemMaxOp && sv.elemMaxOp[e] == 0.f) continue;
+      const float4 lo = sv.eBoxes[2 * (int64_t)e], hi = sv.eBoxes[2 * (int64_t)e + 1];
+      if (!(p.x >= lo.x && p.y >= lo.y && p.z >= lo.z && p.x <= hi.x && p.y <= hi.y && p.z <= hi.z)) continue;
+      float v;
+      const int t = testElement<false>(v, p, sv.indices + 8 * (int64_t)e, sv.vertices);
+      if (t >= 0) {
+        s.primID = e;
+        s.cellID = -1;
+        s.value = v;
+        kind = 1 + t;
+        return s;
+      }
+    }
+  }
+  return s;
+}
+
+// BVH point location; kind receives -1 miss, 0 gridlet, 1..4 tet/pyr/wedge/hex.
+EXA_DEV Sample locateBvh(const SamplerView &sv, const vec3f p, int32_t *stack, const int stride, int &kind) {
   Sample s{-1, -1, 0.f};
   kind = -1;
   if (sv.numGridlets > 0) {
@@ -140,6 +213,12 @@ EXA_DEV Sample locatePoint(const SamplerView &sv, const vec3f p, int32_t *stack,
     if (s.primID >= 0) kind = 1 + bestType;
   }
   return s;
+}
+
+template <int kAccel>
+EXA_DEV Sample locatePoint(const SamplerView &sv, const vec3f p, int32_t *stack, const int stride, int &kind) {
+  if (kAccel == kAccelTables) return locateTables(sv, p, kind);
+  return locateBvh(sv, p, stack, stride, kind);
 }
 
 // sampler/ExaStitchSampler.h:109-126 testSample

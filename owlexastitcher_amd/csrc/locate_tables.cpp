@@ -1,0 +1,138 @@
+// locate_tables.cpp -- host build of the direct point-location tables (the
+// default replacement for the OptiX sampling BVH of
+// sampler/ExaStitchSampler.cpp:71-323).  See LocateTables in exa_host.h.
+#include <math.h>
+#include <algorithm>
+#include <atomic>
+#include <stdexcept>
+#include <thread>
+#include "exa_host.h"
+
+namespace exa {
+
+// bin index of coordinate x: the same float expression runs on the device
+static inline int32_t binOf(float x, float lo, float inv, int32_t dims) {
+  const float f = floorf((x - lo) * inv);
+  int32_t i = f != f ? 0 : (f < 0.f ? 0 : (f > (float)(dims - 1) ? dims - 1 : (int32_t)f));
+  return i;
+}
+
+void buildLocateTables(const HostModel &m, LocateTables &t) {
+  t = LocateTables();
+  // ---- gridlets: per-level macrocell lists --------------------------------
+  int maxLevel = -1;
+  for (const Gridlet &g : m.gridlets) {
+    for (int k = 0; k < 3; ++k)
+      if (g.lower[k] < 0) throw std::runtime_error("locate tables need gridlets with lower >= 0");
+    maxLevel = std::max(maxLevel, g.level);
+  }
+  if (maxLevel >= LocateTables::kMaxLevels) throw std::runtime_error("locate tables: too many levels");
+  t.numLevels = maxLevel + 1;
+  for (const Gridlet &g : m.gridlets)
+    for (int k = 0; k < 3; ++k)
+      t.levelDims[g.level][k] = std::max(t.levelDims[g.level][k], ((g.lower[k] + g.dims[k]) >> 3) + 1);
+  t.levelOffset[0] = 0;
+  for (int L = 0; L < t.numLevels; ++L)
+    t.levelOffset[L + 1] =
+        t.levelOffset[L] + (int64_t)t.levelDims[L][0] * t.levelDims[L][1] * t.levelDims[L][2];
+  const int64_t numMC = t.levelOffset[t.numLevels];
+  if (numMC >= (1ll << 30)) throw std::runtime_error("locate tables: macrocell tables too large");
+  std::vector<uint32_t> cnt(numMC + 1, 0);
+  auto forEachMC = [&](const Gridlet &g, auto &&f) {
+    const int32_t *D = t.levelDims[g.level];
+    const int64_t base = t.levelOffset[g.level];
+    for (int z = g.lower[2] >> 3; z <= (g.lower[2] + g.dims[2]) >> 3; ++z)
+      for (int y = g.lower[1] >> 3; y <= (g.lower[1] + g.dims[1]) >> 3; ++y)
+        for (int x = g.lower[0] >> 3; x <= (g.lower[0] + g.dims[0]) >> 3; ++x)
+          f(base + ((int64_t)z * D[1] + y) * D[0] + x);
+  };
+  for (const Gridlet &g : m.gridlets) forEachMC(g, [&](int64_t i) { cnt[i + 1]++; });
+  for (int64_t i = 0; i < numMC; ++i) cnt[i + 1] += cnt[i];
+  t.gStart = cnt;
+  t.gList.resize(cnt[numMC]);
+  {
+    std::vector<uint32_t> cur(cnt.begin(), cnt.end() - 1);
+    for (size_t g = 0; g < m.gridlets.size(); ++g)
+      forEachMC(m.gridlets[g], [&](int64_t i) { t.gList[cur[i]++] = (int32_t)g; });
+  }
+  // ---- elements: uniform bins -----------------------------------------------
+  const int64_t ne = m.numElements();
+  if (ne == 0) return;
+  std::vector<float> boxes;
+  elementBoxes(m.indices, m.vertices, boxes);
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  std::vector<float> ext(ne);
+  for (int64_t e = 0; e < ne; ++e) {
+    const float *b = &boxes[6 * e];
+    float mx = 0.f;
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = std::min(lo[k], b[k]);
+      hi[k] = std::max(hi[k], b[3 + k]);
+      mx = std::max(mx, b[3 + k] - b[k]);
+    }
+    ext[e] = mx;
+  }
+  std::nth_element(ext.begin(), ext.begin() + ne / 2, ext.end());
+  float size = 1.f;
+  while (size * 2.f <= ext[ne / 2] && size < 1024.f) size *= 2.f;
+  while (size < 1e-3f * std::max(std::max(hi[0] - lo[0], hi[1] - lo[1]), hi[2] - lo[2])) size *= 2.f;
+  for (;;) {
+    for (int k = 0; k < 3; ++k) t.binDims[k] = (int32_t)floorf((hi[k] - lo[k]) * (1.f / size)) + 1;
+    if ((int64_t)t.binDims[0] * t.binDims[1] * t.binDims[2] <= (1ll << 27)) break;
+    size *= 2.f;
+  }
+  for (int k = 0; k < 3; ++k) t.binLo[k] = lo[k];
+  t.binInv = 1.f / size;
+  const int64_t nb = (int64_t)t.binDims[0] * t.binDims[1] * t.binDims[2];
+  std::vector<std::atomic<uint32_t>> bc(nb);
+  for (auto &c : bc) c.store(0, std::memory_order_relaxed);
+  auto range = [&](int64_t e, int32_t a[3], int32_t b[3]) {
+    const float *bx = &boxes[6 * e];
+    for (int k = 0; k < 3; ++k) {
+      a[k] = binOf(bx[k], t.binLo[k], t.binInv, t.binDims[k]);
+      b[k] = binOf(bx[3 + k], t.binLo[k], t.binInv, t.binDims[k]);
+    }
+  };
+  const int T = hostThreads();
+  {
+    std::vector<std::thread> pool;
+    for (int th = 0; th < T; ++th)
+      pool.emplace_back([&, th] {
+        for (int64_t e = th; e < ne; e += T) {
+          int32_t a[3], b[3];
+          range(e, a, b);
+          for (int z = a[2]; z <= b[2]; ++z)
+            for (int y = a[1]; y <= b[1]; ++y)
+              for (int x = a[0]; x <= b[0]; ++x)
+                bc[((int64_t)z * t.binDims[1] + y) * t.binDims[0] + x].fetch_add(1, std::memory_order_relaxed);
+        }
+      });
+    for (auto &p : pool) p.join();
+  }
+  t.eStart.assign(nb + 1, 0);
+  for (int64_t i = 0; i < nb; ++i) t.eStart[i + 1] = t.eStart[i] + bc[i].load(std::memory_order_relaxed);
+  if (t.eStart[nb] >= (1u << 31)) throw std::runtime_error("locate tables: element bin lists too large");
+  t.eList.resize(t.eStart[nb]);
+  std::vector<uint32_t> cur(t.eStart.begin(), t.eStart.end() - 1);
+  for (int64_t e = 0; e < ne; ++e) {
+    int32_t a[3], b[3];
+    range(e, a, b);
+    for (int z = a[2]; z <= b[2]; ++z)
+      for (int y = a[1]; y <= b[1]; ++y)
+        for (int x = a[0]; x <= b[0]; ++x) t.eList[cur[((int64_t)z * t.binDims[1] + y) * t.binDims[0] + x]++] = (int32_t)e;
+  }
+  t.eBoxes.resize(8 * ne);
+  for (int64_t e = 0; e < ne; ++e) {
+    float *o = &t.eBoxes[8 * e];
+    o[0] = boxes[6 * e];
+    o[1] = boxes[6 * e + 1];
+    o[2] = boxes[6 * e + 2];
+    o[3] = 0.f;
+    o[4] = boxes[6 * e + 3];
+    o[5] = boxes[6 * e + 4];
+    o[6] = boxes[6 * e + 5];
+    o[7] = 0.f;
+  }
+}
+
+}  // namespace exa

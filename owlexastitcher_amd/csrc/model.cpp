@@ -5,6 +5,8 @@
 #include <math.h>
 #include <string.h>
 #include <algorithm>
+#include <atomic>
+#include <thread>
 #include <numeric>
 #include <stdexcept>
 #include <string>
@@ -123,7 +125,7 @@ struct Builder {
   int leafSize;
   std::vector<int32_t> ids;
   std::vector<float> cent;  // 3 per prim
-  Bvh *out;
+  std::atomic<int> maxDepth{0};
 
   void boxOf(int64_t b, int64_t e, float lo[3], float hi[3]) const {
     for (int k = 0; k < 3; ++k) { lo[k] = INFINITY; hi[k] = -INFINITY; }
@@ -146,17 +148,16 @@ struct Builder {
     while (c > 1) { c = (c + 1) / 2; ++l; }
     return l;
   }
-
-  // returns the child reference (node index, or first prim for a leaf) and count
-  void build(int64_t b, int64_t e, int depth, int32_t &ref, int32_t &cnt) {
-    const int64_t n = e - b;
-    out->depth = std::max(out->depth, depth);
-    if (n <= leafSize) {
-      std::sort(ids.begin() + b, ids.begin() + e);
-      ref = (int32_t)b;
-      cnt = (int32_t)n;
-      return;
+  void noteDepth(int d) {
+    int cur = maxDepth.load();
+    while (d > cur && !maxDepth.compare_exchange_weak(cur, d)) {
     }
+  }
+
+  // Splits ids[b,e) (binned SAH over 16 bins on each axis; object median when
+  // SAH finds no split or the depth budget requires halving); returns mid.
+  int64_t split(int64_t b, int64_t e, int depth) {
+    const int64_t n = e - b;
     float clo[3] = {INFINITY, INFINITY, INFINITY}, chi[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (int64_t i = b; i < e; ++i)
       for (int k = 0; k < 3; ++k) {
@@ -238,15 +239,50 @@ struct Builder {
         return cp < cq || (cp == cq && p < q);
       });
     }
-    const int32_t nodeIdx = (int32_t)out->nodes.size();
-    out->nodes.emplace_back();
+    return mid;
+  }
+
+  // Builds the subtree over ids[b,e) into `nodes` (indices local to that
+  // vector); returns the child reference (inner node index, or first prim of
+  // a leaf) and the count (0 for an inner node).  Large subtrees near the top
+  // are built on their own threads and appended with their indices shifted.
+  void build(int64_t b, int64_t e, int depth, std::vector<BvhNode> &nodes, int32_t &ref, int32_t &cnt) {
+    const int64_t n = e - b;
+    noteDepth(depth);
+    if (n <= leafSize) {
+      std::sort(ids.begin() + b, ids.begin() + e);
+      ref = (int32_t)b;
+      cnt = (int32_t)n;
+      return;
+    }
+    const int64_t mid = split(b, e, depth);
+    const int32_t nodeIdx = (int32_t)nodes.size();
+    nodes.emplace_back();
     float lo0[3], hi0[3], lo1[3], hi1[3];
     boxOf(b, mid, lo0, hi0);
     boxOf(mid, e, lo1, hi1);
     int32_t r0, c0, r1, c1;
-    build(b, mid, depth + 1, r0, c0);
-    build(mid, e, depth + 1, r1, c1);
-    BvhNode &nd = out->nodes[nodeIdx];
+    if (n > (1 << 16) && depth < 6) {
+      std::vector<BvhNode> left, right;
+      std::thread t([&] { build(b, mid, depth + 1, left, r0, c0); });
+      build(mid, e, depth + 1, right, r1, c1);
+      t.join();
+      auto append = [&](std::vector<BvhNode> &sub, int32_t &r, int32_t c) {
+        const int32_t off = (int32_t)nodes.size();
+        for (BvhNode &nd : sub) {
+          if (nd.cnt0 == 0) nd.ref0 += off;
+          if (nd.cnt1 == 0) nd.ref1 += off;
+        }
+        nodes.insert(nodes.end(), sub.begin(), sub.end());
+        if (c == 0) r += off;
+      };
+      append(left, r0, c0);
+      append(right, r1, c1);
+    } else {
+      build(b, mid, depth + 1, nodes, r0, c0);
+      build(mid, e, depth + 1, nodes, r1, c1);
+    }
+    BvhNode &nd = nodes[nodeIdx];
     for (int k = 0; k < 3; ++k) {
       nd.lo0[k] = lo0[k]; nd.hi0[k] = hi0[k];
       nd.lo1[k] = lo1[k]; nd.hi1[k] = hi1[k];
@@ -269,35 +305,26 @@ void buildBvh(const float *boxes, int64_t n, int leafSize, Bvh &out) {
   Builder B;
   B.boxes = boxes;
   B.leafSize = std::max(1, std::min(leafSize, 16));
-  B.out = &out;
   B.ids.resize(n);
   std::iota(B.ids.begin(), B.ids.end(), 0);
   B.cent.resize(3 * n);
   for (int64_t i = 0; i < n; ++i)
     for (int k = 0; k < 3; ++k) B.cent[3 * i + k] = 0.5f * (boxes[6 * i + k] + boxes[6 * i + 3 + k]);
-  // root: a node whose child 0 covers everything; child 1 empty
-  out.nodes.emplace_back();
-  float lo[3], hi[3];
-  B.boxOf(0, n, lo, hi);
+  // the first inner node built is the root (index 0); a single-leaf tree
+  // gets a root whose child 0 is that leaf and whose child 1 is empty
   int32_t r, c;
-  B.build(0, n, 1, r, c);
+  B.build(0, n, 1, out.nodes, r, c);
+  out.depth = B.maxDepth.load();
   if (c > 0) {
+    float lo[3], hi[3];
+    B.boxOf(0, n, lo, hi);
+    out.nodes.emplace_back();
     BvhNode &root = out.nodes[0];
     for (int k = 0; k < 3; ++k) {
       root.lo0[k] = lo[k]; root.hi0[k] = hi[k];
       root.lo1[k] = INFINITY; root.hi1[k] = -INFINITY;
     }
     root.ref0 = r; root.cnt0 = c;
-    root.ref1 = -1; root.cnt1 = -1;
-  } else {
-    // the recursive build placed the real root at index 1; make node 0 a
-    // pass-through whose only child is that node
-    BvhNode &root = out.nodes[0];
-    for (int k = 0; k < 3; ++k) {
-      root.lo0[k] = lo[k]; root.hi0[k] = hi[k];
-      root.lo1[k] = INFINITY; root.hi1[k] = -INFINITY;
-    }
-    root.ref0 = r; root.cnt0 = 0;
     root.ref1 = -1; root.cnt1 = -1;
   }
   if (out.depth >= kBvhMaxDepth) throw std::runtime_error("buildBvh: depth bound violated");

@@ -79,7 +79,7 @@ struct LocalCounters {
   uint32_t c[kNumCounters];
 };
 
-template <bool kCount>
+template <bool kCount, int kAccel>
 struct Ctx {
   const RenderParams &P;
   const SamplerView &sv;
@@ -100,8 +100,8 @@ struct Interaction {
 
 // deviceCode.cu:699-854: DDA over the majorant grid, Woodcock (shadow=false)
 // or ratio tracking (shadow=true) inside each macrocell.
-template <bool kCount>
-EXA_DEV Interaction sampleInteraction(Ctx<kCount> &C, Ray ray, bool shadow) {
+template <bool kCount, int kAccel>
+EXA_DEV Interaction sampleInteraction(Ctx<kCount, kAccel> &C, Ray ray, bool shadow) {
   const RenderParams &P = C.P;
   ray.org = xfmPoint(P.voxelXfm, ray.org);
   ray.dir = xfmVector(P.voxelXfm, ray.dir);
@@ -162,7 +162,7 @@ EXA_DEV Interaction sampleInteraction(Ctx<kCount> &C, Ray ray, bool shadow) {
           if (kCount) C.cnt.c[kCntCachedHit]++;
         } else {
           int kind;
-          s = locatePoint(C.sv, I.pos, C.stack, kRenderBlock, kind);
+          s = locatePoint<kAccel>(C.sv, I.pos, C.stack, kRenderBlock, kind);
           if (kCount) C.cnt.c[kind < 0 ? kCntMiss : kCntGridletHit + kind]++;
         }
         if (s.primID < 0) continue;
@@ -214,8 +214,8 @@ EXA_DEV Interaction sampleInteraction(Ctx<kCount> &C, Ray ray, bool shadow) {
 
 // deviceCode.cu:856-926 sampleLight (+ directional lights, no reference
 // counterpart: unit falloff, shadow ray to the domain exit)
-template <bool kCount>
-EXA_DEV vec3f sampleLight(Ctx<kCount> &C, vec3f pos, int numLights) {
+template <bool kCount, int kAccel>
+EXA_DEV vec3f sampleLight(Ctx<kCount, kAccel> &C, vec3f pos, int numLights) {
   const RenderParams &P = C.P;
   int which = (int)(C.rng() * (float)numLights);
   if (which == numLights) which = 0;
@@ -245,8 +245,8 @@ EXA_DEV vec3f sampleLight(Ctx<kCount> &C, vec3f pos, int numLights) {
 }
 
 // deviceCode.cu:144-149, 965-985 (g = 0)
-template <bool kCount>
-EXA_DEV vec3f hgSample(Ctx<kCount> &C, vec3f wo) {
+template <bool kCount, int kAccel>
+EXA_DEV vec3f hgSample(Ctx<kCount, kAccel> &C, vec3f wo) {
   const float u1 = C.rng();
   const float u2 = C.rng();
   const float cost = 1.f - 2.f * u1;
@@ -260,8 +260,8 @@ EXA_DEV vec3f hgSample(Ctx<kCount> &C, vec3f wo) {
   return (sint * cp) * u + (sint * sp) * v + cost * (-w);
 }
 
-template <bool kCount>
-EXA_DEV float4 pathTrace(Ctx<kCount> &C, Ray ray, float4 bg, int numLights) {
+template <bool kCount, int kAccel>
+EXA_DEV float4 pathTrace(Ctx<kCount, kAccel> &C, Ray ray, float4 bg, int numLights) {
   const RenderParams &P = C.P;
   vec3f throughput = mk3s(1.f);
   vec3f Ld = mk3s(0.f);
@@ -303,8 +303,8 @@ EXA_DEV float4 pathTrace(Ctx<kCount> &C, Ray ray, float4 bg, int numLights) {
   return bounce ? make_float4(L.x, L.y, L.z, 1.f) : bg;
 }
 
-template <bool kCount>
-EXA_DEV float4 directLight(Ctx<kCount> &C, Ray ray, float4 bg, int numLights) {
+template <bool kCount, int kAccel>
+EXA_DEV float4 directLight(Ctx<kCount, kAccel> &C, Ray ray, float4 bg, int numLights) {
   const RenderParams &P = C.P;
   vec3f throughput = mk3s(1.f);
   float t0 = 1e30f, t1 = -1e30f;
@@ -327,10 +327,10 @@ EXA_DEV uint32_t make8bit(float f) {
   return (uint32_t)(i < 0 ? 0 : (i > 255 ? 255 : i));
 }
 
-template <bool kCount>
+template <bool kCount, int kAccel>
 __global__ __launch_bounds__(kRenderBlock) void kRender(RenderParams P, SamplerView sv, GridView gv,
                                                         unsigned long long *counters) {
-  __shared__ int32_t stack[kStackDepth * kRenderBlock];
+  __shared__ int32_t stack[kAccel == kAccelBvh ? kStackDepth * kRenderBlock : 1];
   const int tid = threadIdx.y * kTileX + threadIdx.x;
   const int px = blockIdx.x * kTileX + threadIdx.x;
   const int py = blockIdx.y * kTileY + threadIdx.y;
@@ -341,7 +341,7 @@ __global__ __launch_bounds__(kRenderBlock) void kRender(RenderParams P, SamplerV
   Lcg rng;
   const int pixelID = px + P.W * py;
   rng.init((uint32_t)pixelID, (uint32_t)P.accumID);
-  Ctx<kCount> C{P, sv, gv, rng, stack + tid, cnt};
+  Ctx<kCount, kAccel> C{P, sv, gv, rng, stack + tid, cnt};
   const float t = (float)py / (float)P.H;
   const float4 bg = make_float4((1.f - t) * 1.f + t * 0.5f, (1.f - t) * 1.f + t * 0.7f, (1.f - t) * 1.f + t * 1.f,
                                 1.f);
@@ -420,14 +420,17 @@ __global__ __launch_bounds__(kRenderBlock) void kRender(RenderParams P, SamplerV
   }
 }
 
-void launchRender(const RenderParams &p, const SamplerView &sv, const GridView &gv, unsigned long long *counters,
-                  hipStream_t s) {
+void launchRender(const RenderParams &p, const SamplerView &sv, int accel, const GridView &gv,
+                  unsigned long long *counters, hipStream_t s) {
   const dim3 grid((p.W + kTileX - 1) / kTileX, (p.H + kTileY - 1) / kTileY);
   const dim3 block(kTileX, kTileY);
-  if (counters)
-    hipLaunchKernelGGL(kRender<true>, grid, block, 0, s, p, sv, gv, counters);
-  else
-    hipLaunchKernelGGL(kRender<false>, grid, block, 0, s, p, sv, gv, counters);
+  if (accel == kAccelBvh) {
+    if (counters) hipLaunchKernelGGL((kRender<true, kAccelBvh>), grid, block, 0, s, p, sv, gv, counters);
+    else hipLaunchKernelGGL((kRender<false, kAccelBvh>), grid, block, 0, s, p, sv, gv, counters);
+  } else {
+    if (counters) hipLaunchKernelGGL((kRender<true, kAccelTables>), grid, block, 0, s, p, sv, gv, counters);
+    else hipLaunchKernelGGL((kRender<false, kAccelTables>), grid, block, 0, s, p, sv, gv, counters);
+  }
 }
 
 }  // namespace exa

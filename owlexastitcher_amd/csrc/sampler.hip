@@ -238,16 +238,16 @@ __global__ void kElementMaxOpacity(const float4 *__restrict__ vertices, const in
   out[e] = mapMaxOpacity(lo, hi, cmap, numColors, x0, x1);
 }
 
-template <bool kCount>
+template <bool kCount, int kAccel>
 __global__ __launch_bounds__(kLocateBlock) void kLocate(SamplerView sv, const float *__restrict__ pos,
                                                         int64_t n, Sample *__restrict__ out,
                                                         unsigned long long *counters) {
-  __shared__ int32_t stack[kStackDepth * kLocateBlock];
+  __shared__ int32_t stack[kAccel == kAccelBvh ? kStackDepth * kLocateBlock : 1];
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= n) return;
   const vec3f p = mk3(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
   int kind;
-  const Sample s = locatePoint(sv, p, stack + threadIdx.x, kLocateBlock, kind);
+  const Sample s = locatePoint<kAccel>(sv, p, stack + threadIdx.x, kLocateBlock, kind);
   out[i] = s;
   if (kCount) atomicAdd(&counters[kind < 0 ? kCntMiss : kCntGridletHit + kind], 1ull);
 }
@@ -299,15 +299,17 @@ void launchElementMaxOpacity(const float4 *vertices, const int32_t *indices, int
                      cmap, numColors, x0, x1, out);
 }
 
-void launchLocate(const SamplerView &sv, const float *pos, int64_t n, Sample *out,
+void launchLocate(const SamplerView &sv, int accel, const float *pos, int64_t n, Sample *out,
                   unsigned long long *counters, hipStream_t s) {
   if (n <= 0) return;
-  if (counters)
-    hipLaunchKernelGGL(kLocate<true>, dim3(blocksFor(n, kLocateBlock)), dim3(kLocateBlock), 0, s, sv, pos, n,
-                       out, counters);
-  else
-    hipLaunchKernelGGL(kLocate<false>, dim3(blocksFor(n, kLocateBlock)), dim3(kLocateBlock), 0, s, sv, pos,
-                       n, out, counters);
+  const dim3 g(blocksFor(n, kLocateBlock)), b(kLocateBlock);
+  if (accel == kAccelBvh) {
+    if (counters) hipLaunchKernelGGL((kLocate<true, kAccelBvh>), g, b, 0, s, sv, pos, n, out, counters);
+    else hipLaunchKernelGGL((kLocate<false, kAccelBvh>), g, b, 0, s, sv, pos, n, out, counters);
+  } else {
+    if (counters) hipLaunchKernelGGL((kLocate<true, kAccelTables>), g, b, 0, s, sv, pos, n, out, counters);
+    else hipLaunchKernelGGL((kLocate<false, kAccelTables>), g, b, 0, s, sv, pos, n, out, counters);
+  }
 }
 
 void launchTestSample(const SamplerView &sv, const float *pos, const int32_t *primIDs, int64_t n, Sample *out,

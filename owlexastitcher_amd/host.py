@@ -16,10 +16,13 @@ from ._capi import (AmrCell, Gridlet, RenderState, StitchModelDesc, check, lib)
 
 PATH_TRACER = 0
 DIRECT_LIGHTING = 1
+ACCEL_TABLES = 0
+ACCEL_BVH = 1
 
 
 def _ptr(a):
-    return C.c_void_p(a.ctypes.data) if a is not None else None
+    # data_as keeps a reference to the array for the lifetime of the pointer
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
 
 
 def _f32(x, n=None):
@@ -179,6 +182,13 @@ class ExaStitchSampler:
         check(lib().exa_sampler_build(self.h, model.h, _stream(stream)))
         return True
 
+    def setAccel(self, accel):
+        check(lib().exa_sampler_set_accel(self.h, int(accel)))
+
+    @property
+    def accel(self):
+        return int(lib().exa_sampler_get_accel(self.h))
+
     def computeMaxOpacities(self, rgba, xf_range, stream=None):
         cm = _f32(rgba).reshape(-1, 4)
         check(lib().exa_sampler_compute_max_opacities(self.h, _ptr(cm), len(cm), float(xf_range[0]),
@@ -329,6 +339,9 @@ class OWLRenderer:
     def setMaxBounces(self, n):
         check(lib().exa_renderer_set_max_bounces(self.h, int(n)))
 
+    def setAccel(self, accel):
+        check(lib().exa_renderer_set_accel(self.h, int(accel)))
+
     def setDebugCrosshair(self, on):
         check(lib().exa_renderer_set_debug_crosshair(self.h, int(bool(on))))
 
@@ -347,6 +360,10 @@ class OWLRenderer:
     @property
     def accumID(self):
         return int(lib().exa_renderer_get_accum_id(self.h))
+
+    @accumID.setter
+    def accumID(self, v):
+        check(lib().exa_renderer_set_accum_id(self.h, int(v)))
 
     def state(self):
         s = RenderState()

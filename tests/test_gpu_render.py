@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as ora
-from owlexastitcher_amd import OWLRenderer, camera_look_at
+from owlexastitcher_amd import ACCEL_BVH, ACCEL_TABLES, OWLRenderer, camera_look_at
 from tests import scenes
 
 pytestmark = pytest.mark.gpu
@@ -68,10 +68,12 @@ def _report(a, b):
     return f"differing pixels {npx}/{a.shape[0] * a.shape[1]}, max abs diff {d.max():.3g}"
 
 
+@pytest.mark.parametrize("accel", [ACCEL_TABLES, ACCEL_BVH], ids=["tables", "bvh"])
 @pytest.mark.parametrize("integrator,light", [(0, "dir"), (1, "point"), (0, "none")])
-def test_render_matches_oracle_f32(cfg1, device, integrator, light):
-    print(f"case: render config1 64x48 integrator={integrator} light={light}", flush=True)
+def test_render_matches_oracle_f32(cfg1, device, integrator, light, accel):
+    print(f"case: render config1 64x48 integrator={integrator} light={light} accel={accel}", flush=True)
     r = make_renderer(cfg1, 64, 48)
+    r.setAccel(accel)
     r.setType(integrator)
     r.setMaxBounces(4)
     b = r.modelBounds

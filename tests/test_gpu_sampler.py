@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as ora
-from owlexastitcher_amd import ExaStitchSampler
+from owlexastitcher_amd import ACCEL_BVH, ACCEL_TABLES, ExaStitchSampler
 from tests import scenes
 
 pytestmark = pytest.mark.gpu
@@ -21,11 +21,13 @@ def cfg1():
     return scenes.config1()
 
 
-@pytest.fixture(scope="module")
-def sampler1(cfg1, device):
+@pytest.fixture(scope="module", params=[ACCEL_TABLES, ACCEL_BVH], ids=["tables", "bvh"])
+def sampler1(request, cfg1, device):
     cfg1.model.setNumGridCells((32, 32, 32))
     s = ExaStitchSampler()
+    s.setAccel(request.param)
     s.build(cfg1.model)
+    assert s.accel == request.param
     return s
 
 
@@ -159,3 +161,30 @@ def test_test_sample(cfg1, sampler1, device):
             assert out[i, 2:3].view(np.float32)[0] == v[0]
         else:
             assert out[i, 0] == -1 and out[i, 1] == -1
+
+
+@pytest.fixture(scope="module")
+def noise_scene():
+    return scenes.noise_small()
+
+
+@pytest.mark.parametrize("accel", [ACCEL_TABLES, ACCEL_BVH], ids=["tables", "bvh"])
+def test_locate_noise_scene(noise_scene, device, accel):
+    """3-level scene (pyramids, wedges, hexes at two level transitions)."""
+    sc = noise_scene
+    sc.model.setNumGridCells((16, 16, 16))
+    s = ExaStitchSampler()
+    s.setAccel(accel)
+    s.build(sc.model)
+    lo, hi = sc.cell_bounds[:3], sc.cell_bounds[3:]
+    rng = np.random.default_rng(23)
+    pos = rng.uniform(lo - 1.0, hi + 1.0, size=(1 << 18, 3)).astype(np.float32)
+    ids, vals = _locate_gpu(s, pos, device)
+    om = ora.Model(sc.verts, sc.idx, sc.gridlets, sc.gscalars)
+    ref = om.locate_binned(pos)
+    mism, rel, _ = _compare(ids, vals, ref)
+    assert mism == 0 and rel <= VALUE_RTOL
+    # element hits of every kind present
+    kinds = set(np.unique(np.count_nonzero(sc.idx[ref[ref[:, 1] < 0][:, 0][ref[ref[:, 1] < 0][:, 0] >= 0]] >= 0,
+                                           axis=1)))
+    assert {5, 6, 8} <= kinds
