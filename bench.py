@@ -78,6 +78,9 @@ def setup_renderer(model, args):
     r.setMaxBounces(args.bounces)
     r.setSpp(args.spp)
     r.setDirectionalLight(0, (0.3, 1.0, 0.5), 2.0, True)
+    from owlexastitcher_amd import RENDER_MEGAKERNEL, RENDER_PERSISTENT
+    r.setScheme(RENDER_PERSISTENT if args.scheme == "persistent" else RENDER_MEGAKERNEL)
+    r.setPersistentTuning(args.ctrl_steps, args.elem_batch, args.rare_batch)
     if args.accel == "bvh":
         from owlexastitcher_amd import ACCEL_BVH
         t = time.time()
@@ -153,6 +156,10 @@ def main():
     ap.add_argument("--mcs", type=int, default=64)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--accel", choices=["tables", "bvh"], default="tables")
+    ap.add_argument("--scheme", choices=["persistent", "megakernel"], default="persistent")
+    ap.add_argument("--ctrl-steps", type=int, default=8)
+    ap.add_argument("--elem-batch", type=int, default=16)
+    ap.add_argument("--rare-batch", type=int, default=16)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -230,12 +237,15 @@ def main():
                          + ("path tracer <= %d bounces" % args.bounces if args.integrator == 0
                             else "single scatter") + ", 1 directional light"),
             "accel": args.accel,
+            "scheme": args.scheme,
+            "persistent_tuning": [args.ctrl_steps, args.elem_batch, args.rare_batch],
             "frame_ms": kernel_ms,
             "volume_samples_per_s": queries / (kernel_ms * 1e-3),
             "counters": cnt,
         },
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "kRender<false>"},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "kRenderPersistent" if args.scheme == "persistent" else "kRender"},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

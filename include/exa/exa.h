@@ -190,6 +190,21 @@ const exa_sampler *exa_renderer_get_sampler(const exa_renderer *r);
 /* selects the sampler's accelerator (building it from the renderer's model
    copy when needed) */
 int exa_renderer_set_accel(exa_renderer *r, int accel);
+/* Launch scheme of render(); both give bit-identical images (no reference
+   counterpart: the reference launches one OptiX raygen thread per pixel,
+   deviceCode.cu:1268-1360).
+   EXA_RENDER_MEGAKERNEL: one thread per pixel, 16x16 tiles.
+   EXA_RENDER_PERSISTENT (default): persistent lanes that take pixels from a
+   work counter and run their paths as a state machine, in per-wave passes.
+   Tuning: ctrlSteps = free-flight / macrocell transitions per lane and pass
+   (>= 1); elemBatch and rareBatch in [0, 64] = wave fraction (x/64) of live
+   lanes that must wait for an element test, resp. for per-segment work,
+   before that pass runs.  Defaults 8, 16, 16. */
+#define EXA_RENDER_MEGAKERNEL 0
+#define EXA_RENDER_PERSISTENT 1
+int exa_renderer_set_scheme(exa_renderer *r, int scheme);
+int exa_renderer_get_scheme(const exa_renderer *r);
+int exa_renderer_set_persistent_tuning(exa_renderer *r, int ctrlSteps, int elemBatch, int rareBatch);
 
 /* Snapshot of the launch parameters of the next render() call (for checking
    against the CPU oracle). */

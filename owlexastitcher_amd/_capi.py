@@ -133,6 +133,9 @@ _SIGS = {
     "exa_renderer_set_accum_id": (C.c_int, [P, C.c_int]),
     "exa_renderer_get_sampler": (P, [P]),
     "exa_renderer_set_accel": (C.c_int, [P, C.c_int]),
+    "exa_renderer_set_scheme": (C.c_int, [P, C.c_int]),
+    "exa_renderer_get_scheme": (C.c_int, [P]),
+    "exa_renderer_set_persistent_tuning": (C.c_int, [P, C.c_int, C.c_int, C.c_int]),
     "exa_renderer_get_state": (C.c_int, [P, P]),
     "exa_renderer_destroy": (None, [P]),
     "exa_camera_look_at": (C.c_int, [P, P, P, F, C.c_int, C.c_int, P, P, P, P]),

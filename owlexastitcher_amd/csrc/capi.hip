@@ -158,8 +158,8 @@ struct exa_sampler {
   int gDepth = 0, eDepth = 0;
   LocateTables tabMeta;  // sizes/offsets only (vectors are released after upload)
   DeviceBuffer<uint32_t> tGStart, tEStart;
-  DeviceBuffer<int32_t> tGList, tEList;
-  DeviceBuffer<float4> tEBoxes;
+  DeviceBuffer<int32_t> tGList, tGTop, tGOwner;
+  DeviceBuffer<float4> tECand, tEVerts;
   // gridlet value ranges + per-primitive max opacities (culling)
   DeviceBuffer<float2> gridletRanges;
   DeviceBuffer<float> gridletMaxOp, elemMaxOp;
@@ -194,9 +194,11 @@ struct exa_sampler {
       for (int k = 0; k < 3; ++k) v.levelDims[L][k] = tabMeta.levelDims[L][k];
       v.levelOffset[L] = (int32_t)tabMeta.levelOffset[L];
     }
+    v.gTop = tGTop.count ? tGTop.ptr : nullptr;
+    v.gOwner = tGOwner.ptr;
     v.eStart = tEStart.ptr;
-    v.eList = tEList.ptr;
-    v.eBoxes = tEBoxes.ptr;
+    v.eCand = tECand.ptr;
+    v.eVerts = tEVerts.ptr;
     for (int k = 0; k < 3; ++k) {
       v.binLo[k] = tabMeta.binLo[k];
       v.binDims[k] = tabMeta.binDims[k];
@@ -228,9 +230,11 @@ struct exa_sampler {
     buildLocateTables(m, t);
     tGStart.upload(t.gStart.data(), t.gStart.size(), s);
     tGList.upload(t.gList.data(), t.gList.size(), s);
+    tGTop.upload(t.gTop.data(), t.gTop.size(), s);
+    tGOwner.upload(t.gOwner.data(), t.gOwner.size(), s);
     tEStart.upload(t.eStart.data(), t.eStart.size(), s);
-    tEList.upload(t.eList.data(), t.eList.size(), s);
-    tEBoxes.upload((const float4 *)t.eBoxes.data(), t.eBoxes.size() / 4, s);
+    tECand.upload((const float4 *)t.eCand.data(), t.eCand.size() / 4, s);
+    tEVerts.upload((const float4 *)t.eVerts.data(), t.eVerts.size() / 4, s);
     HIP_CHECK(hipStreamSynchronize(s));
     tabMeta = LocateTables();
     tabMeta.numLevels = t.numLevels;
@@ -344,6 +348,9 @@ struct exa_renderer {
   float valueRange[2];
   int accumID = 0;
   int spp = 1;
+  int scheme = EXA_RENDER_PERSISTENT;
+  PersistTuning tuning;
+  DeviceBuffer<uint32_t> work;
 
   void updateMaxOpacities() {
     if (colorMap.empty()) return;
@@ -613,8 +620,8 @@ int exa_sampler_get_info(const exa_sampler *s, int64_t info[8]) {
     info[5] = (int64_t)s->eRefs.count;
     info[6] = (int64_t)(s->vertices.bytes() + s->indices.bytes() + s->gridlets.bytes() + s->gridletScalars.bytes());
     info[7] = (int64_t)(s->gNodes.bytes() + s->eNodes.bytes() + s->gRefs.bytes() + s->eRefs.bytes() +
-                        s->tGStart.bytes() + s->tGList.bytes() + s->tEStart.bytes() + s->tEList.bytes() +
-                        s->tEBoxes.bytes());
+                        s->tGStart.bytes() + s->tGList.bytes() + s->tGTop.bytes() + s->tGOwner.bytes() +
+                        s->tEStart.bytes() + s->tECand.bytes() + s->tEVerts.bytes());
   });
 }
 int exa_sampler_set_accel(exa_sampler *s, int accel) {
@@ -713,7 +720,13 @@ static void doRender(exa_renderer *r, uint32_t *fb, unsigned long long *counters
   P.spp = r->spp < 1 ? 1 : r->spp;
   P.tf = r->tf.ptr;
   P.tfN = (int32_t)r->tf.count;
-  launchRender(P, r->sampler.view(), r->sampler.accel, r->sampler.gridView(), counters, s);
+  if (r->scheme == EXA_RENDER_PERSISTENT) {
+    r->work.alloc(1);
+    launchRenderPersistent(P, r->sampler.view(), r->sampler.accel, r->sampler.gridView(), r->work.ptr, r->tuning,
+                           counters, s);
+  } else {
+    launchRender(P, r->sampler.view(), r->sampler.accel, r->sampler.gridView(), counters, s);
+  }
   HIP_CHECK(hipGetLastError());
   r->accumID++;
 }
@@ -887,6 +900,25 @@ int exa_renderer_set_accel(exa_renderer *r, int accel) {
   return guarded([&] {
     require(r, "renderer is NULL");
     r->sampler.setAccel(accel, &r->model, 0);
+  });
+}
+int exa_renderer_set_scheme(exa_renderer *r, int scheme) {
+  return guarded([&] {
+    require(r, "renderer is NULL");
+    require(scheme == EXA_RENDER_MEGAKERNEL || scheme == EXA_RENDER_PERSISTENT, "unknown render scheme");
+    r->scheme = scheme;
+  });
+}
+int exa_renderer_get_scheme(const exa_renderer *r) { return r ? r->scheme : EXA_ERR_INVALID; }
+int exa_renderer_set_persistent_tuning(exa_renderer *r, int ctrlSteps, int elemBatch, int rareBatch) {
+  return guarded([&] {
+    require(r, "renderer is NULL");
+    require(ctrlSteps >= 1 && ctrlSteps <= 1024, "ctrlSteps must be in [1, 1024]");
+    require(elemBatch >= 0 && elemBatch <= 64, "elemBatch must be in [0, 64]");
+    require(rareBatch >= 0 && rareBatch <= 64, "rareBatch must be in [0, 64]");
+    r->tuning.ctrlSteps = ctrlSteps;
+    r->tuning.elemBatch = elemBatch;
+    r->tuning.rareBatch = rareBatch;
   });
 }
 int exa_renderer_get_state(const exa_renderer *r, exa_render_state *o) {

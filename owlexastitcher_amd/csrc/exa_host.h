@@ -110,17 +110,26 @@ void gridletBoxes(const std::vector<Gridlet> &g, std::vector<float> &boxes);
 // element whose AABB contains a point is in that point's bin.
 struct LocateTables {
   static constexpr int kMaxLevels = 16;
+  static constexpr int kOwnerBlock = 512;  // 8^3 dual cells per macrocell
   int32_t numLevels = 0;                 // levels 0..numLevels-1 (may be empty)
   int32_t levelDims[kMaxLevels][3] = {};  // macrocells per axis
   int64_t levelOffset[kMaxLevels + 1] = {};
   std::vector<uint32_t> gStart;  // per macrocell (all levels concatenated) + 1
   std::vector<int32_t> gList;
+  // Dual-cell owner tables (empty when a coordinate reaches 8192, where the
+  // interior test below loses its rounding margin): per macrocell the index
+  // of its owner block or -1; per block, for each of its 8^3 dual cells, the
+  // lowest-ID gridlet whose cell there has 8 non-NaN corners, or -1.
+  std::vector<int32_t> gTop;
+  std::vector<int32_t> gOwner;
   float binLo[3] = {0, 0, 0};
   float binInv = 1.f;  // 1 / bin size (a power of two)
   int32_t binDims[3] = {0, 0, 0};
   std::vector<uint32_t> eStart;  // per bin + 1
-  std::vector<int32_t> eList;
-  std::vector<float> eBoxes;  // 8 per element: lo.xyz, 0, hi.xyz, 0
+  // one record per list entry: lo.xyz + element id (int bits), hi.xyz + vertex count
+  std::vector<float> eCand;
+  // per element its vertices (xyz + scalar) in index order, 8 slots
+  std::vector<float> eVerts;
 };
 void buildLocateTables(const HostModel &m, LocateTables &t);
 void elementBoxes(const std::vector<int32_t> &indices, const std::vector<float> &vertices,

@@ -30,9 +30,11 @@ struct SamplerView {
   int32_t numLevels;
   int32_t levelDims[LocateTables::kMaxLevels][3];
   int32_t levelOffset[LocateTables::kMaxLevels];
+  const int32_t *gTop;    // nullptr: owner tables not built
+  const int32_t *gOwner;
   const uint32_t *eStart;
-  const int32_t *eList;
-  const float4 *eBoxes;
+  const float4 *eCand;   // 2 per list entry: lo.xyz + id, hi.xyz + vertex count
+  const float4 *eVerts;  // 8 per element
   float binLo[3];
   float binInv;
   int32_t binDims[3];
@@ -120,5 +122,15 @@ struct RenderParams {
 
 void launchRender(const RenderParams &p, const SamplerView &sv, int accel, const GridView &gv,
                   unsigned long long *counters, hipStream_t s);
+// scheduling knobs of the persistent renderer (results do not depend on them)
+struct PersistTuning {
+  int32_t ctrlSteps = 8;   // tracking transitions per lane and round, >= 1
+  int32_t elemBatch = 16;  // element pass when waiting lanes >= elemBatch/64 of live lanes
+  int32_t rareBatch = 16;  // per-segment pass when waiting lanes >= rareBatch/64 of live lanes
+};
+// work: one device uint32 (reset by the launch itself)
+void launchRenderPersistent(const RenderParams &p, const SamplerView &sv, int accel, const GridView &gv,
+                            uint32_t *work, const PersistTuning &t, unsigned long long *counters,
+                            hipStream_t s);
 
 }  // namespace exa

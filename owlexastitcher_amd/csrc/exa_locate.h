@@ -89,22 +89,73 @@ EXA_DEV int testElement(float &value, vec3f P, const int32_t *__restrict__ idx8,
   return nv == 4 ? 0 : (nv == 5 ? 1 : (nv == 6 ? 2 : 3));
 }
 
-// Direct-table point location (LocateTables): per level, the gridlet list of
-// macrocell floor((p/2^L - .5)/8); then the element list of p's bin.
-EXA_DEV Sample locateTables(const SamplerView &sv, const vec3f p, int &kind) {
+// Element test on an element's own vertex record (8 float4 slots holding the
+// same values as vertices[indices[...]]); same arithmetic as testElement.
+EXA_DEV int testElementRecord(float &value, vec3f P, int nv, const float4 *__restrict__ rec) {
+  float4 v[8];
+  v[0] = rec[0];
+  v[1] = rec[1];
+  v[2] = rec[2];
+  v[3] = rec[3];
+  if (nv > 4) v[4] = rec[4];
+  if (nv > 5) v[5] = rec[5];
+  if (nv > 6) { v[6] = rec[6]; v[7] = rec[7]; }
+  bool hit;
+  if (nv == 4) hit = intersectTet(value, P, v[0], v[1], v[2], v[3]);
+  else if (nv == 5) hit = intersectNewton<5>(value, P, v);
+  else if (nv == 6) hit = intersectNewton<6>(value, P, v);
+  else if (nv == 8) hit = intersectNewton<8>(value, P, v);
+  else hit = false;
+  if (!hit) return -1;
+  return nv == 4 ? 0 : (nv == 5 ? 1 : (nv == 6 ? 2 : 3));
+}
+
+// Direct-table point location (LocateTables), gridlet stage: per level, the
+// gridlet list of macrocell floor((p/2^L - .5)/8).  A hit strictly inside a
+// gridlet cell ends the search (dual cells of one mesh tile space, so no
+// lower-ID gridlet can also hold the point); a boundary hit only bounds the
+// IDs still worth testing.
+EXA_DEV Sample locateTablesGridlets(const SamplerView &sv, const vec3f p) {
   Sample s{-1, -1, 0.f};
-  kind = -1;
   int32_t best = 0x7fffffff;
   for (int L = 0; L < sv.numLevels; ++L) {
     const int32_t *D = sv.levelDims[L];
     const float inv = 1.f / (float)(1 << L);
-    const float mxf = floorf((p.x * inv - 0.5f) * 0.125f);
-    const float myf = floorf((p.y * inv - 0.5f) * 0.125f);
-    const float mzf = floorf((p.z * inv - 0.5f) * 0.125f);
+    const float gx = p.x * inv - 0.5f, gy = p.y * inv - 0.5f, gz = p.z * inv - 0.5f;
+    const float mxf = floorf(gx * 0.125f);
+    const float myf = floorf(gy * 0.125f);
+    const float mzf = floorf(gz * 0.125f);
     if (!(mxf >= 0.f && myf >= 0.f && mzf >= 0.f && mxf < (float)D[0] && myf < (float)D[1] &&
           mzf < (float)D[2]))
       continue;
     const int32_t idx = sv.levelOffset[L] + ((int32_t)mzf * D[1] + (int32_t)myf) * D[0] + (int32_t)mxf;
+    if (sv.gTop) {
+      // Owner-table fast path for points at least 2^-7 of a cell away from
+      // every dual-cell face of this level (coordinates < 8192, so the
+      // gridlet test sees fractions in [2^-8, 1-2^-8], a band-2 hit).  There
+      // the lowest-ID accepting gridlet of the level is the cell's owner.
+      const float cx = floorf(gx), cy = floorf(gy), cz = floorf(gz);
+      const float fx = gx - cx, fy = gy - cy, fz = gz - cz;
+      const float m0 = 0.0078125f, m1 = 0.9921875f;
+      if (fx >= m0 && fx <= m1 && fy >= m0 && fy <= m1 && fz >= m0 && fz <= m1) {
+        const int32_t blk = sv.gTop[idx];
+        if (blk < 0) continue;
+        const int32_t local = (((int32_t)cz & 7) * 8 + ((int32_t)cy & 7)) * 8 + ((int32_t)cx & 7);
+        const int32_t g = sv.gOwner[(int64_t)blk * LocateTables::kOwnerBlock + local];
+        if (g < 0) continue;
+        if (!(sv.gridletMaxOp && sv.gridletMaxOp[g] == 0.f)) {
+          float v;
+          int cid;
+          if (intersectGridlet(v, cid, p, sv.gridlets[g], sv.gridletScalars) == 2) {
+            s.primID = g;
+            s.cellID = cid;
+            s.value = v;
+            return s;
+          }
+        }
+        // culled owner: the list scan below finds the next accepting gridlet
+      }
+    }
     const uint32_t k1 = sv.gStart[idx + 1];
     for (uint32_t k = sv.gStart[idx]; k < k1; ++k) {
       const int32_t g = sv.gList[k];
@@ -118,35 +169,48 @@ EXA_DEV Sample locateTables(const SamplerView &sv, const vec3f p, int &kind) {
         s.primID = g;
         s.cellID = cid;
         s.value = v;
-        if (r == 2) {
-          kind = 0;
-          return s;
-        }
+        if (r == 2) return s;
         break;
       }
     }
   }
-  if (s.primID >= 0) {
-    kind = 0;
-    return s;
-  }
-  if (sv.numElements > 0) {
-    int32_t b[3];
-    const float q[3] = {p.x, p.y, p.z};
+  return s;
+}
+
+// Direct-table point location, element stage: the element list of p's bin,
+// stored AABB first; kind receives 1..4 (tet/pyr/wedge/hex) or -1.
+EXA_DEV Sample locateTablesElements(const SamplerView &sv, const vec3f p, int &kind) {
+  Sample s{-1, -1, 0.f};
+  kind = -1;
+  if (sv.numElements <= 0) return s;
+  int32_t b[3];
+  const float q[3] = {p.x, p.y, p.z};
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      const float f = floorf((q[k] - sv.binLo[k]) * sv.binInv);
-      b[k] = f != f ? 0 : (f < 0.f ? 0 : (f > (float)(sv.binDims[k] - 1) ? sv.binDims[k] - 1 : (int32_t)f));
-    }
-    const int64_t bin = ((int64_t)b[2] * sv.binDims[1] + b[1]) * sv.binDims[0] + b[0];
-    const uint32_t k1 = sv.eStart[bin + 1];
-    for (uint32_t k = sv.eStart[bin]; k < k1; ++k) {
-      const int32_t e = sv.eList[k];
+  for (int k = 0; k < 3; ++k) {
+    const float f = floorf((q[k] - sv.binLo[k]) * sv.binInv);
+    b[k] = f != f ? 0 : (f < 0.f ? 0 : (f > (float)(sv.binDims[k] - 1) ? sv.binDims[k] - 1 : (int32_t)f));
+  }
+  const int64_t bin = ((int64_t)b[2] * sv.binDims[1] + b[1]) * sv.binDims[0] + b[0];
+  const uint32_t k0 = sv.eStart[bin], k1 = sv.eStart[bin + 1];
+  // candidates in ID order, four records per batch of independent loads
+  for (uint32_t k = k0; k < k1; k += 4) {
+    float4 lo[4], hi[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      if (k + j < k1) {
+        lo[j] = sv.eCand[2 * (size_t)(k + j)];
+        hi[j] = sv.eCand[2 * (size_t)(k + j) + 1];
+      }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      if (k + j >= k1) break;
+      if (!(p.x >= lo[j].x && p.y >= lo[j].y && p.z >= lo[j].z && p.x <= hi[j].x && p.y <= hi[j].y &&
+            p.z <= hi[j].z))
+        continue;
+      const int32_t e = __float_as_int(lo[j].w);
       if (sv.elemMaxOp && sv.elemMaxOp[e] == 0.f) continue;
-      const float4 lo = sv.eBoxes[2 * (int64_t)e], hi = sv.eBoxes[2 * (int64_t)e + 1];
-      if (!(p.x >= lo.x && p.y >= lo.y && p.z >= lo.z && p.x <= hi.x && p.y <= hi.y && p.z <= hi.z)) continue;
       float v;
-      const int t = testElement<false>(v, p, sv.indices + 8 * (int64_t)e, sv.vertices);
+      const int t = testElementRecord(v, p, __float_as_int(hi[j].w), sv.eVerts + 8 * (int64_t)e);
       if (t >= 0) {
         s.primID = e;
         s.cellID = -1;
@@ -159,66 +223,84 @@ EXA_DEV Sample locateTables(const SamplerView &sv, const vec3f p, int &kind) {
   return s;
 }
 
-// BVH point location; kind receives -1 miss, 0 gridlet, 1..4 tet/pyr/wedge/hex.
-EXA_DEV Sample locateBvh(const SamplerView &sv, const vec3f p, int32_t *stack, const int stride, int &kind) {
+// BVH point location, gridlet stage.
+EXA_DEV Sample locateBvhGridlets(const SamplerView &sv, const vec3f p, int32_t *stack, const int stride) {
   Sample s{-1, -1, 0.f};
-  kind = -1;
-  if (sv.numGridlets > 0) {
-    int32_t best = 0x7fffffff;
-    bvhPointQuery(sv.gNodes, p, stack, stride, [&](int32_t first, int32_t count) -> bool {
-      for (int32_t k = 0; k < count; ++k) {
-        const int32_t g = sv.gRefs[first + k];
-        if (g >= best) break;  // refs ascend within a leaf
-        if (sv.gridletMaxOp && sv.gridletMaxOp[g] == 0.f) continue;
-        float v;
-        int cid;
-        const int r = intersectGridlet(v, cid, p, sv.gridlets[g], sv.gridletScalars);
-        if (r) {
-          best = g;
-          s.primID = g;
-          s.cellID = cid;
-          s.value = v;
-          if (r == 2) return true;
-          break;
-        }
+  if (sv.numGridlets <= 0) return s;
+  int32_t best = 0x7fffffff;
+  bvhPointQuery(sv.gNodes, p, stack, stride, [&](int32_t first, int32_t count) -> bool {
+    for (int32_t k = 0; k < count; ++k) {
+      const int32_t g = sv.gRefs[first + k];
+      if (g >= best) break;  // refs ascend within a leaf
+      if (sv.gridletMaxOp && sv.gridletMaxOp[g] == 0.f) continue;
+      float v;
+      int cid;
+      const int r = intersectGridlet(v, cid, p, sv.gridlets[g], sv.gridletScalars);
+      if (r) {
+        best = g;
+        s.primID = g;
+        s.cellID = cid;
+        s.value = v;
+        if (r == 2) return true;
+        break;
       }
-      return false;
-    });
-    if (s.primID >= 0) {
-      kind = 0;
-      return s;
     }
-  }
-  if (sv.numElements > 0) {
-    int32_t best = 0x7fffffff;
-    int bestType = -1;
-    bvhPointQuery(sv.eNodes, p, stack, stride, [&](int32_t first, int32_t count) -> bool {
-      for (int32_t k = 0; k < count; ++k) {
-        const int32_t e = sv.eRefs[first + k];
-        if (e >= best) break;
-        if (sv.elemMaxOp && sv.elemMaxOp[e] == 0.f) continue;
-        float v;
-        const int t = testElement(v, p, sv.indices + 8 * (int64_t)e, sv.vertices);
-        if (t >= 0) {
-          best = e;
-          bestType = t;
-          s.primID = e;
-          s.cellID = -1;
-          s.value = v;
-          break;
-        }
-      }
-      return false;
-    });
-    if (s.primID >= 0) kind = 1 + bestType;
-  }
+    return false;
+  });
   return s;
 }
 
+// BVH point location, element stage; kind receives 1..4 or -1.
+EXA_DEV Sample locateBvhElements(const SamplerView &sv, const vec3f p, int32_t *stack, const int stride,
+                                 int &kind) {
+  Sample s{-1, -1, 0.f};
+  kind = -1;
+  if (sv.numElements <= 0) return s;
+  int32_t best = 0x7fffffff;
+  int bestType = -1;
+  bvhPointQuery(sv.eNodes, p, stack, stride, [&](int32_t first, int32_t count) -> bool {
+    for (int32_t k = 0; k < count; ++k) {
+      const int32_t e = sv.eRefs[first + k];
+      if (e >= best) break;
+      if (sv.elemMaxOp && sv.elemMaxOp[e] == 0.f) continue;
+      float v;
+      const int t = testElement(v, p, sv.indices + 8 * (int64_t)e, sv.vertices);
+      if (t >= 0) {
+        best = e;
+        bestType = t;
+        s.primID = e;
+        s.cellID = -1;
+        s.value = v;
+        break;
+      }
+    }
+    return false;
+  });
+  if (s.primID >= 0) kind = 1 + bestType;
+  return s;
+}
+
+// The two stages of point location, by accelerator.
+template <int kAccel>
+EXA_DEV Sample locateGridlets(const SamplerView &sv, const vec3f p, int32_t *stack, const int stride) {
+  if (kAccel == kAccelTables) return locateTablesGridlets(sv, p);
+  return locateBvhGridlets(sv, p, stack, stride);
+}
+template <int kAccel>
+EXA_DEV Sample locateElements(const SamplerView &sv, const vec3f p, int32_t *stack, const int stride, int &kind) {
+  if (kAccel == kAccelTables) return locateTablesElements(sv, p, kind);
+  return locateBvhElements(sv, p, stack, stride, kind);
+}
+
+// Full point location; kind receives -1 miss, 0 gridlet, 1..4 tet/pyr/wedge/hex.
 template <int kAccel>
 EXA_DEV Sample locatePoint(const SamplerView &sv, const vec3f p, int32_t *stack, const int stride, int &kind) {
-  if (kAccel == kAccelTables) return locateTables(sv, p, kind);
-  return locateBvh(sv, p, stack, stride, kind);
+  const Sample s = locateGridlets<kAccel>(sv, p, stack, stride);
+  if (s.primID >= 0) {
+    kind = 0;
+    return s;
+  }
+  return locateElements<kAccel>(sv, p, stack, stride, kind);
 }
 
 // sampler/ExaStitchSampler.h:109-126 testSample

@@ -2,6 +2,7 @@
 // default replacement for the OptiX sampling BVH of
 // sampler/ExaStitchSampler.cpp:71-323).  See LocateTables in exa_host.h.
 #include <math.h>
+#include <string.h>
 #include <algorithm>
 #include <atomic>
 #include <stdexcept>
@@ -15,6 +16,58 @@ static inline int32_t binOf(float x, float lo, float inv, int32_t dims) {
   const float f = floorf((x - lo) * inv);
   int32_t i = f != f ? 0 : (f < 0.f ? 0 : (f > (float)(dims - 1) ? dims - 1 : (int32_t)f));
   return i;
+}
+
+// Dual-cell owner tables.  Gridlet IDs ascend with level, so per level the
+// owner of a dual cell is the first gridlet (in ID order) that has a valid
+// cell there; blocks are allocated for macrocells that hold any valid cell.
+static void buildOwnerTables(const HostModel &m, LocateTables &t) {
+  for (const Gridlet &g : m.gridlets)
+    for (int k = 0; k < 3; ++k)
+      if ((int64_t)(g.lower[k] + g.dims[k] + 1) << g.level >= 8192) return;
+  const int64_t numMC = t.levelOffset[t.numLevels];
+  const float *S = m.gridletScalars.data();
+  auto valid = [&](const Gridlet &g, int x, int y, int z) {
+    const int sx = g.dims[0] + 1, sy = g.dims[1] + 1;
+    const float *s = S + g.begin;
+    for (int dz = 0; dz < 2; ++dz)
+      for (int dy = 0; dy < 2; ++dy)
+        for (int dx = 0; dx < 2; ++dx) {
+          const float f = s[((int64_t)(z + dz) * sy + (y + dy)) * sx + (x + dx)];
+          if (f != f) return false;
+        }
+    return true;
+  };
+  auto mcOf = [&](const Gridlet &g, int cx, int cy, int cz) {
+    const int32_t *D = t.levelDims[g.level];
+    return t.levelOffset[g.level] + ((int64_t)(cz >> 3) * D[1] + (cy >> 3)) * D[0] + (cx >> 3);
+  };
+  t.gTop.assign(numMC, -1);
+  int64_t numBlocks = 0;
+  for (const Gridlet &g : m.gridlets)
+    for (int z = 0; z < g.dims[2]; ++z)
+      for (int y = 0; y < g.dims[1]; ++y)
+        for (int x = 0; x < g.dims[0]; ++x) {
+          const int64_t mc = mcOf(g, g.lower[0] + x, g.lower[1] + y, g.lower[2] + z);
+          if (t.gTop[mc] < 0 && valid(g, x, y, z)) t.gTop[mc] = (int32_t)numBlocks++;
+        }
+  if (numBlocks * LocateTables::kOwnerBlock >= (1ll << 31)) {
+    t.gTop.clear();
+    return;
+  }
+  t.gOwner.assign(numBlocks * LocateTables::kOwnerBlock, -1);
+  for (size_t gi = 0; gi < m.gridlets.size(); ++gi) {
+    const Gridlet &g = m.gridlets[gi];
+    for (int z = 0; z < g.dims[2]; ++z)
+      for (int y = 0; y < g.dims[1]; ++y)
+        for (int x = 0; x < g.dims[0]; ++x) {
+          const int cx = g.lower[0] + x, cy = g.lower[1] + y, cz = g.lower[2] + z;
+          const int32_t blk = t.gTop[mcOf(g, cx, cy, cz)];
+          if (blk < 0) continue;
+          int32_t &o = t.gOwner[(int64_t)blk * LocateTables::kOwnerBlock + ((cz & 7) * 8 + (cy & 7)) * 8 + (cx & 7)];
+          if (o < 0 && valid(g, x, y, z)) o = (int32_t)gi;
+        }
+  }
 }
 
 void buildLocateTables(const HostModel &m, LocateTables &t) {
@@ -55,6 +108,7 @@ void buildLocateTables(const HostModel &m, LocateTables &t) {
     for (size_t g = 0; g < m.gridlets.size(); ++g)
       forEachMC(m.gridlets[g], [&](int64_t i) { t.gList[cur[i]++] = (int32_t)g; });
   }
+  buildOwnerTables(m, t);
   // ---- elements: uniform bins -----------------------------------------------
   const int64_t ne = m.numElements();
   if (ne == 0) return;
@@ -112,27 +166,36 @@ void buildLocateTables(const HostModel &m, LocateTables &t) {
   t.eStart.assign(nb + 1, 0);
   for (int64_t i = 0; i < nb; ++i) t.eStart[i + 1] = t.eStart[i] + bc[i].load(std::memory_order_relaxed);
   if (t.eStart[nb] >= (1u << 31)) throw std::runtime_error("locate tables: element bin lists too large");
-  t.eList.resize(t.eStart[nb]);
+  t.eCand.resize(8 * (size_t)t.eStart[nb]);
   std::vector<uint32_t> cur(t.eStart.begin(), t.eStart.end() - 1);
   for (int64_t e = 0; e < ne; ++e) {
     int32_t a[3], b[3];
     range(e, a, b);
+    const int32_t *row = &m.indices[8 * e];
+    int32_t nv = 0;
+    while (nv < 8 && row[nv] >= 0) nv++;
+    float rec[8];
+    const int32_t eid = (int32_t)e;
+    rec[0] = boxes[6 * e];
+    rec[1] = boxes[6 * e + 1];
+    rec[2] = boxes[6 * e + 2];
+    memcpy(&rec[3], &eid, 4);
+    rec[4] = boxes[6 * e + 3];
+    rec[5] = boxes[6 * e + 4];
+    rec[6] = boxes[6 * e + 5];
+    memcpy(&rec[7], &nv, 4);
     for (int z = a[2]; z <= b[2]; ++z)
       for (int y = a[1]; y <= b[1]; ++y)
-        for (int x = a[0]; x <= b[0]; ++x) t.eList[cur[((int64_t)z * t.binDims[1] + y) * t.binDims[0] + x]++] = (int32_t)e;
+        for (int x = a[0]; x <= b[0]; ++x)
+          memcpy(&t.eCand[8 * (size_t)cur[((int64_t)z * t.binDims[1] + y) * t.binDims[0] + x]++], rec, 32);
   }
-  t.eBoxes.resize(8 * ne);
-  for (int64_t e = 0; e < ne; ++e) {
-    float *o = &t.eBoxes[8 * e];
-    o[0] = boxes[6 * e];
-    o[1] = boxes[6 * e + 1];
-    o[2] = boxes[6 * e + 2];
-    o[3] = 0.f;
-    o[4] = boxes[6 * e + 3];
-    o[5] = boxes[6 * e + 4];
-    o[6] = boxes[6 * e + 5];
-    o[7] = 0.f;
-  }
+  t.eVerts.assign(32 * (size_t)ne, 0.f);
+  for (int64_t e = 0; e < ne; ++e)
+    for (int i = 0; i < 8; ++i) {
+      const int32_t v = m.indices[8 * e + i];
+      if (v < 0) break;
+      memcpy(&t.eVerts[32 * (size_t)e + 4 * i], &m.vertices[4 * (size_t)v], 16);
+    }
 }
 
 }  // namespace exa

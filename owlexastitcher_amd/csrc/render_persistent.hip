@@ -1,0 +1,386 @@
+// render_persistent.hip -- persistent path-regeneration renderer for gfx950.
+//
+// The megakernel (render.hip) gives every thread one pixel, so a wave runs
+// as long as its longest path and executes every branch of the integrator
+// that any of its 64 lanes takes.  Here each lane runs its path as a
+// resumable state machine, and a wave runs the parts of the machine in
+// phases, so that lanes doing the same kind of work run it together:
+//  - rare pass: the per-segment work (end of a segment and the integrator
+//    step after it, phase-function sampling, camera rays, segment set-up);
+//    runs when at least rareBatch/64 of the live lanes wait for it;
+//  - tracking pass: free-flight steps and DDA macrocell steps, up to
+//    ctrlSteps per lane, until each lane reaches a tentative collision;
+//  - gridlet pass: cached-gridlet test, then the gridlet stage of point
+//    location;
+//  - element pass: the element stage (Newton iterations over pyramids,
+//    wedges, hexes) for lanes whose gridlet stage missed; runs when at least
+//    elemBatch/64 of the live lanes wait for it;
+//  - resolve pass: TF lookup and the Woodcock / ratio-tracking decision.
+// When no lane can make progress in the tracking, gridlet or resolve passes,
+// both gated passes run regardless of their thresholds, so every wave
+// drains.  A lane that finishes its pixel takes the next one from a
+// frame-wide work counter (one atomic per wave and pass); pixels are handed
+// out in 8x8 tiles in scan order.
+// Each lane draws random numbers and does its floating-point work in exactly
+// the order of the megakernel (both use the helpers of exa_render_common.h),
+// so every pixel's result is bit-identical to it; only the interleaving of
+// lanes changes.
+#include "exa_render_common.h"
+
+namespace exa {
+
+constexpr int kPersistBlock = 256;
+// minimum waves per SIMD the register allocation must allow (spills beyond)
+#ifndef EXA_PERSIST_MIN_WAVES
+#define EXA_PERSIST_MIN_WAVES 2
+#endif
+constexpr int kPixelTile = 8;  // 8x8 pixels per 64-lane grab
+
+enum Stage : int32_t {
+  // tracking pass
+  kStStep = 0,  // free-flight step inside the macrocell
+  kStCellEnd,   // DDA step to the next macrocell
+  kStCellBegin, // enter a macrocell
+  // point location and resolution
+  kStAtSample,  // collision position ready for point location
+  kStPending,   // gridlet stage missed: waits for the element pass
+  // rare pass
+  kStSegEnd,       // segment done: continue the integrator
+  kStScatter,      // phase-function sample, next path ray
+  kStGrab,         // take the next pixel
+  kStSampleStart,  // camera ray of the next sample
+  kStPtSegment,    // path tracer: begin the segment along the path ray
+  kStDone
+};
+enum SegKind : int32_t { kSegPt = 0, kSegPtShadow, kSegDl, kSegDlShadow };
+
+EXA_DEV bool isTracking(int st) { return st <= kStCellBegin; }
+EXA_DEV bool isRare(int st) { return st >= kStSegEnd && st < kStDone; }
+
+struct PathLane {
+  int32_t st, kind, px, py, s;
+  Lcg rng;
+  float4 acc;
+  // integrator state (pathTracingIntegrate / directLightingIntegrate); the
+  // direct-lighting integrator keeps the albedo of its interaction in Ld
+  vec3f throughput, Ld;
+  unsigned bounce;
+  Ray ray;     // path ray; ray.tmax is pathTracingIntegrate's t1
+  vec3f spos;  // world-space position of the last interaction
+  int32_t which;
+  float ld;
+  // current segment (sampleInteraction)
+  Segment S;
+  float t1, majorant, t, Tr;
+  int32_t primID, itype;
+  vec3f pos;  // voxel-space collision position
+  float4 xf;
+};
+
+struct PersistParams {
+  uint32_t *work;     // frame-wide pixel-tile counter, zero at launch
+  uint32_t numWork;   // tilesX * tilesY * 64
+  int32_t tilesX;
+  int32_t ctrlSteps;  // tracking transitions per lane and round
+  int32_t elemBatch;  // element pass runs when pending*64 >= live*elemBatch
+  int32_t rareBatch;  // rare pass runs when rare*64 >= live*rareBatch
+};
+
+EXA_DEV void beginSegment(PathLane &L, const RenderParams &P, const GridView &gv, const Ray &ray, int kind) {
+  L.kind = kind;
+  segmentBegin(P, gv, ray, L.S);
+  L.Tr = 1.f;
+  L.itype = kBoundary;
+  L.pos = mk3s(0.f);
+  L.xf = make_float4(0.f, 0.f, 0.f, 0.f);
+  L.st = kStCellBegin;
+}
+
+// renderFrame_Impl's sample loop (deviceCode.cu:1283-1316): accumulate the
+// sample; after spp samples, write the pixel and ask for the next one.
+template <bool kCount>
+EXA_DEV void endSample(PathLane &L, const RenderParams &P, float4 c, LocalCounters &cnt) {
+  L.acc.x += c.x;
+  L.acc.y += c.y;
+  L.acc.z += c.z;
+  L.acc.w += c.w;
+  L.s++;
+  if (L.s < P.spp) {
+    L.st = kStSampleStart;
+    return;
+  }
+  finishPixel(P, L.px, L.py, L.acc);
+  if (kCount) cnt.c[kCntPixels]++;
+  L.st = kStGrab;
+}
+
+// tracking pass transition: Woodcock / ratio-tracking free flight and dda3
+template <bool kCount>
+EXA_DEV void trackingTransition(PathLane &L, const RenderParams &P, const GridView &gv, LocalCounters &cnt) {
+  if (L.st == kStStep) {
+    if (L.majorant <= 0.f || !trackingStep(L.t, L.majorant, P.opacityScale, L.t1, L.rng)) {
+      L.st = kStCellEnd;
+    } else {
+      L.pos = L.S.org + L.S.dir * L.t;
+      L.st = kStAtSample;
+    }
+  } else if (L.st == kStCellEnd) {
+    L.st = segmentNextCell(gv, L.S, L.t1) ? kStCellBegin : kStSegEnd;
+  } else {
+    L.t1 = segmentCellExit(L.S);
+    if (kCount) cnt.c[kCntDdaCells]++;
+    L.majorant = gv.majorants[segmentCellID(gv, L.S)];
+    L.t = L.S.t0;
+    L.primID = -1;
+    L.st = kStStep;
+  }
+}
+
+// end of a segment: the rest of the integrator step that traced it
+template <bool kCount>
+EXA_DEV void segmentEnd(PathLane &L, const RenderParams &P, const GridView &gv, int numLights,
+                        LocalCounters &cnt) {
+  const vec3f wpos = xfmPoint(P.voxelXfmInv, L.pos);
+  if (L.kind == kSegPt) {
+    if (L.itype == kBoundary || !ptInteraction(P, L.xf, L.throughput, L.bounce, L.rng)) {
+      endSample<kCount>(L, P, ptResult(L.Ld, L.throughput, L.bounce, numLights, backgroundColor(L.py, P.H)), cnt);
+      return;
+    }
+    L.spos = wpos;
+    if (numLights > 0) {
+      const Ray r = lightRay(P, wpos, numLights, L.rng, L.which, L.ld);
+      beginSegment(L, P, gv, r, kSegPtShadow);
+    } else {
+      L.st = kStScatter;
+    }
+  } else if (L.kind == kSegPtShadow) {
+    L.Ld = L.Ld + L.throughput * lightResult(P, L.which, L.Tr, L.ld);
+    L.st = kStScatter;
+  } else if (L.kind == kSegDl) {
+    if (L.itype == kBoundary) {
+      endSample<kCount>(L, P, backgroundColor(L.py, P.H), cnt);
+      return;
+    }
+    const vec3f albedo = mk3(L.xf.x, L.xf.y, L.xf.z);
+    if (numLights > 0) {
+      L.Ld = albedo;
+      const Ray r = lightRay(P, wpos, numLights, L.rng, L.which, L.ld);
+      beginSegment(L, P, gv, r, kSegDlShadow);
+    } else {
+      const vec3f th = mk3s(1.f) * albedo;
+      endSample<kCount>(L, P, make_float4(th.x, th.y, th.z, 1.f), cnt);
+    }
+  } else {
+    const vec3f th = mk3s(1.f) * (L.Ld * lightResult(P, L.which, L.Tr, L.ld));
+    endSample<kCount>(L, P, make_float4(th.x, th.y, th.z, 1.f), cnt);
+  }
+}
+
+// pathTracingIntegrate after sampleLight: new direction, new path ray
+EXA_DEV void scatter(PathLane &L, const RenderParams &P) {
+  const vec3f dir = hgSample(-L.ray.dir, L.rng);
+  L.ray.org = L.spos;
+  L.ray.dir = dir;
+  float t0, t1;
+  boxTest(L.ray, P.worldBounds, t0, t1);
+  clipInterval(P, L.ray, t0, t1);
+  L.ray.tmax = t1;
+  L.st = kStPtSegment;
+}
+
+// camera ray and the integrators' entry (deviceCode.cu:1000-1027, 1130-1150)
+template <bool kCount>
+EXA_DEV void sampleStart(PathLane &L, const RenderParams &P, const GridView &gv, int numLights,
+                         LocalCounters &cnt) {
+  Ray ray = primaryRay(P, L.px, L.py, L.rng);
+  float t0 = 1e30f, t1 = -1e30f;
+  if (P.integrator == 1) {
+    if (!boxTest(ray, P.worldBounds, t0, t1)) {
+      endSample<kCount>(L, P, backgroundColor(L.py, P.H), cnt);
+      return;
+    }
+    clipInterval(P, ray, t0, t1);
+    ray.org = ray.org + ray.dir * t0;
+    t1 -= t0;
+    ray.tmax = t1;
+    beginSegment(L, P, gv, ray, kSegDl);
+  } else {
+    L.throughput = mk3s(1.f);
+    L.Ld = mk3s(0.f);
+    L.bounce = 0;
+    if (!boxTest(ray, P.worldBounds, t0, t1)) {
+      endSample<kCount>(L, P, ptResult(L.Ld, L.throughput, L.bounce, numLights, backgroundColor(L.py, P.H)),
+                        cnt);
+      return;
+    }
+    clipInterval(P, ray, t0, t1);
+    ray.org = ray.org + ray.dir * t0;
+    t1 -= t0;
+    ray.tmax = t1;
+    L.ray = ray;
+    L.st = kStPtSegment;
+  }
+}
+
+// hand out pixels to the lanes that wait for one: one atomic per wave
+EXA_DEV void grabPixels(PathLane &L, const RenderParams &P, const PersistParams &pp, int lane) {
+  const uint64_t want = __ballot(L.st == kStGrab);
+  if (!want) return;
+  const int leader = __ffsll((unsigned long long)want) - 1;
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(pp.work, (uint32_t)__popcll(want));
+  base = __shfl(base, leader);
+  if (L.st != kStGrab) return;
+  const uint32_t w = base + (uint32_t)__popcll(want & ((1ull << lane) - 1ull));
+  if (w >= pp.numWork) {
+    L.st = kStDone;
+    return;
+  }
+  const uint32_t tile = w >> 6, in = w & 63u;
+  const int px = (int)(tile % (uint32_t)pp.tilesX) * kPixelTile + (int)(in & 7u);
+  const int py = (int)(tile / (uint32_t)pp.tilesX) * kPixelTile + (int)(in >> 3);
+  if (px >= P.W || py >= P.H) return;  // outside a partial edge tile: grab again
+  L.px = px;
+  L.py = py;
+  L.rng.init((uint32_t)(px + P.W * py), (uint32_t)P.accumID);
+  L.acc = make_float4(0.f, 0.f, 0.f, 0.f);
+  L.s = 0;
+  L.st = kStSampleStart;
+}
+
+template <bool kCount, int kAccel>
+__global__ __launch_bounds__(kPersistBlock) __attribute__((amdgpu_waves_per_eu(EXA_PERSIST_MIN_WAVES, 8))) void
+kRenderPersistent(RenderParams P, SamplerView sv, GridView gv,
+                                                                   PersistParams pp,
+                                                                   unsigned long long *counters) {
+  __shared__ int32_t stackMem[kAccel == kAccelBvh ? kStackDepth * kPersistBlock : 1];
+  int32_t *stack = stackMem + threadIdx.x;
+  const int lane = (int)__lane_id();
+  LocalCounters cnt;
+  if (kCount)
+    for (int i = 0; i < kNumCounters; ++i) cnt.c[i] = 0;
+  const int numLights = countLights(P);
+  PathLane L;
+  L.st = kStGrab;
+  for (;;) {
+    const uint64_t live = __ballot(L.st != kStDone);
+    if (!live) break;
+    const int nLive = __popcll(live);
+    // lanes that can progress without the gated passes
+    const bool stuck = __ballot(isTracking(L.st) || L.st == kStAtSample) == 0;
+
+    // rare pass, in the order a lane flows through it
+    const uint64_t rare = __ballot(isRare(L.st));
+    if (rare && (stuck || __popcll(rare) * 64 >= nLive * pp.rareBatch)) {
+      if (L.st == kStSegEnd) segmentEnd<kCount>(L, P, gv, numLights, cnt);
+      if (L.st == kStScatter) scatter(L, P);
+      grabPixels(L, P, pp, lane);
+      if (L.st == kStSampleStart) sampleStart<kCount>(L, P, gv, numLights, cnt);
+      if (L.st == kStPtSegment) beginSegment(L, P, gv, L.ray, kSegPt);
+    }
+
+    // tracking pass
+    for (int n = 0; n < pp.ctrlSteps; ++n) {
+      if (!isTracking(L.st)) break;
+      trackingTransition<kCount>(L, P, gv, cnt);
+    }
+
+    // gridlet pass (cached gridlet first)
+    Sample smp{-1, -1, 0.f};
+    bool have = false;
+    if (L.st == kStAtSample) {
+      smp = testSampleGridlet(sv, L.pos, L.primID);
+      if (smp.primID >= 0) {
+        if (kCount) cnt.c[kCntCachedHit]++;
+        have = true;
+      } else {
+        smp = locateGridlets<kAccel>(sv, L.pos, stack, kPersistBlock);
+        if (smp.primID >= 0) {
+          if (kCount) cnt.c[kCntGridletHit]++;
+          have = true;
+        } else {
+          L.st = kStPending;
+        }
+      }
+    }
+
+    // element pass
+    const uint64_t pend = __ballot(L.st == kStPending);
+    if (pend && (stuck || __popcll(pend) * 64 >= nLive * pp.elemBatch)) {
+      if (L.st == kStPending) {
+        int kind;
+        smp = locateElements<kAccel>(sv, L.pos, stack, kPersistBlock, kind);
+        if (kCount) cnt.c[kind < 0 ? kCntMiss : kCntGridletHit + kind]++;
+        have = true;
+      }
+    }
+
+    // resolve pass (Woodcock accept / ratio-tracking update)
+    if (have) {
+      if (kCount && smp.primID >= 0) cnt.c[kCntTfFetch]++;
+      const bool shadow = L.kind == kSegPtShadow || L.kind == kSegDlShadow;
+      if (resolveCollision(P, smp, shadow, L.majorant, L.rng, L.primID, L.xf, L.Tr)) {
+        L.itype = kScattering;
+        L.st = kStSegEnd;
+      } else {
+        L.st = kStStep;
+      }
+    }
+  }
+  if (kCount)
+    for (int i = 0; i < kNumCounters; ++i)
+      if (cnt.c[i]) atomicAdd(&counters[i], (unsigned long long)cnt.c[i]);
+}
+
+__global__ void kResetWork(uint32_t *work) {
+  if (threadIdx.x == 0) *work = 0u;
+}
+
+template <bool kCount, int kAccel>
+static int residentBlocks() {
+  static int cached = 0;
+  if (!cached) {
+    int perCU = 0, dev = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, kRenderPersistent<kCount, kAccel>, kPersistBlock,
+                                                     0) != hipSuccess ||
+        perCU < 1)
+      perCU = 1;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+      cus = 256;
+    cached = perCU * cus;
+  }
+  return cached;
+}
+
+template <bool kCount, int kAccel>
+static void launchPersistentT(const RenderParams &p, const SamplerView &sv, const GridView &gv,
+                              const PersistParams &pp, unsigned long long *counters, hipStream_t s) {
+  const int blocks = std::max(1, std::min(residentBlocks<kCount, kAccel>(),
+                                          (int)((pp.numWork + kPersistBlock - 1) / kPersistBlock)));
+  hipLaunchKernelGGL((kRenderPersistent<kCount, kAccel>), dim3(blocks), dim3(kPersistBlock), 0, s, p, sv, gv, pp,
+                     counters);
+}
+
+void launchRenderPersistent(const RenderParams &p, const SamplerView &sv, int accel, const GridView &gv,
+                            uint32_t *work, const PersistTuning &t, unsigned long long *counters,
+                            hipStream_t s) {
+  PersistParams pp;
+  pp.work = work;
+  pp.tilesX = (p.W + kPixelTile - 1) / kPixelTile;
+  const uint32_t tilesY = (uint32_t)((p.H + kPixelTile - 1) / kPixelTile);
+  pp.numWork = (uint32_t)pp.tilesX * tilesY * 64u;
+  pp.ctrlSteps = t.ctrlSteps < 1 ? 1 : t.ctrlSteps;
+  pp.elemBatch = t.elemBatch < 0 ? 0 : (t.elemBatch > 64 ? 64 : t.elemBatch);
+  pp.rareBatch = t.rareBatch < 0 ? 0 : (t.rareBatch > 64 ? 64 : t.rareBatch);
+  hipLaunchKernelGGL(kResetWork, dim3(1), dim3(64), 0, s, work);
+  if (accel == kAccelBvh) {
+    if (counters) launchPersistentT<true, kAccelBvh>(p, sv, gv, pp, counters, s);
+    else launchPersistentT<false, kAccelBvh>(p, sv, gv, pp, counters, s);
+  } else {
+    if (counters) launchPersistentT<true, kAccelTables>(p, sv, gv, pp, counters, s);
+    else launchPersistentT<false, kAccelTables>(p, sv, gv, pp, counters, s);
+  }
+}
+
+}  // namespace exa

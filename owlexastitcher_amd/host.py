@@ -18,6 +18,8 @@ PATH_TRACER = 0
 DIRECT_LIGHTING = 1
 ACCEL_TABLES = 0
 ACCEL_BVH = 1
+RENDER_MEGAKERNEL = 0
+RENDER_PERSISTENT = 1
 
 
 def _ptr(a):
@@ -341,6 +343,21 @@ class OWLRenderer:
 
     def setAccel(self, accel):
         check(lib().exa_renderer_set_accel(self.h, int(accel)))
+
+    def setScheme(self, scheme):
+        """RENDER_PERSISTENT (default) or RENDER_MEGAKERNEL; same images."""
+        check(lib().exa_renderer_set_scheme(self.h, int(scheme)))
+
+    @property
+    def scheme(self):
+        rc = lib().exa_renderer_get_scheme(self.h)
+        if rc < 0:
+            check(rc)
+        return rc
+
+    def setPersistentTuning(self, ctrlSteps=8, elemBatch=16, rareBatch=16):
+        """Scheduling knobs of RENDER_PERSISTENT; images do not depend on them."""
+        check(lib().exa_renderer_set_persistent_tuning(self.h, int(ctrlSteps), int(elemBatch), int(rareBatch)))
 
     def setDebugCrosshair(self, on):
         check(lib().exa_renderer_set_debug_crosshair(self.h, int(bool(on))))

@@ -1,4 +1,5 @@
-"""GPU parity of the raygen megakernel against the single-precision oracle.
+"""GPU parity of the renderers (megakernel and persistent) against the
+single-precision oracle.
 
 With the same per-pixel seed (pixelID, accumID), the same LCG<4> stream and
 the same operation order, the accumulation buffer is compared bit for bit.
@@ -7,7 +8,8 @@ import numpy as np
 import pytest
 
 from oracle import oracle as ora
-from owlexastitcher_amd import ACCEL_BVH, ACCEL_TABLES, OWLRenderer, camera_look_at
+from owlexastitcher_amd import (ACCEL_BVH, ACCEL_TABLES, RENDER_MEGAKERNEL, RENDER_PERSISTENT, OWLRenderer,
+                                camera_look_at)
 from tests import scenes
 
 pytestmark = pytest.mark.gpu
@@ -68,12 +70,19 @@ def _report(a, b):
     return f"differing pixels {npx}/{a.shape[0] * a.shape[1]}, max abs diff {d.max():.3g}"
 
 
+SCHEMES = [RENDER_PERSISTENT, RENDER_MEGAKERNEL]
+
+
+@pytest.mark.parametrize("scheme", SCHEMES, ids=["persistent", "megakernel"])
 @pytest.mark.parametrize("accel", [ACCEL_TABLES, ACCEL_BVH], ids=["tables", "bvh"])
 @pytest.mark.parametrize("integrator,light", [(0, "dir"), (1, "point"), (0, "none")])
-def test_render_matches_oracle_f32(cfg1, device, integrator, light, accel):
-    print(f"case: render config1 64x48 integrator={integrator} light={light} accel={accel}", flush=True)
+def test_render_matches_oracle_f32(cfg1, device, integrator, light, accel, scheme):
+    print(f"case: render config1 64x48 integrator={integrator} light={light} accel={accel} scheme={scheme}",
+          flush=True)
     r = make_renderer(cfg1, 64, 48)
     r.setAccel(accel)
+    r.setScheme(scheme)
+    assert r.scheme == scheme
     r.setType(integrator)
     r.setMaxBounces(4)
     b = r.modelBounds
@@ -88,3 +97,49 @@ def test_render_matches_oracle_f32(cfg1, device, integrator, light, accel):
     # the image is not trivial: some pixels scattered in the volume
     bgmask = np.all(acc == acc[:, :1], axis=(1, 2))
     assert not bgmask.all()
+
+
+@pytest.mark.parametrize("tuning", [(1, 0, 0), (1, 64, 64), (64, 32, 0), (3, 17, 64), (5, 0, 40)])
+def test_persistent_tuning_matches_oracle(cfg1, device, tuning):
+    """Images do not depend on the persistent renderer's scheduling knobs;
+    odd sizes exercise partial 8x8 pixel tiles; 3 spp; clip plane on."""
+    print(f"case: persistent tuning {tuning}, 45x29, 3 spp, clip", flush=True)
+    r = make_renderer(cfg1, 45, 29)
+    r.setScheme(RENDER_PERSISTENT)
+    r.setPersistentTuning(*tuning)
+    r.setSpp(3)
+    r.setMaxBounces(4)
+    r.setDirectionalLight(0, (0.3, 1.0, 0.5), 2.0, True)
+    b = r.modelBounds
+    r.setClipPlane(0, True, (0.0, 0.0, 1.0), float(0.5 * (b[2] + b[5])) + 7.0)
+    acc, acc_o, fb, fb_o = render_both(cfg1, r, 1, device)
+    assert np.array_equal(acc, acc_o), _report(acc, acc_o)
+    assert np.array_equal(fb, fb_o)
+
+
+def test_persistent_matches_megakernel_noise(device):
+    """3-level scene, element-heavy (pyramids, wedges, hexes), 160x120: the
+    two launch schemes give identical frames, counters included."""
+    import torch
+    print("case: persistent vs megakernel, noise scene 160x120", flush=True)
+    sc = scenes.noise_small()
+    r = make_renderer(sc, 160, 120, num_mcs=(16, 16, 16), opacity=125.0)
+    r.setMaxBounces(6)
+    r.setDirectionalLight(0, (0.3, 1.0, 0.5), 2.0, True)
+    r.setLightSource(1, r.modelBounds[3:] + 5.0, 4.0e4, True)
+    fb = torch.zeros((120, 160), dtype=torch.int32, device=device)
+    out = {}
+    for scheme in SCHEMES:
+        r.setScheme(scheme)
+        r.accumID = 0
+        r.render(fb)
+        c1 = r.render_counted(fb)
+        torch.cuda.synchronize()
+        out[scheme] = (r.accum(), fb.cpu().numpy().copy(), c1)
+    a0, f0, c0 = out[RENDER_PERSISTENT]
+    a1, f1, c1 = out[RENDER_MEGAKERNEL]
+    assert np.array_equal(a0, a1), _report(a0, a1)
+    assert np.array_equal(f0, f1)
+    assert c0 == c1, (c0, c1)
+    assert c0["hex"] > 0 and c0["wedge"] > 0 and c0["pyr"] > 0
+    assert c0["pixels"] == 160 * 120
