@@ -144,8 +144,10 @@ EXA_HD int intersectGridlet(float &value, int &cellID, vec3f pos, const Gridlet 
         pos.z <= hi.z))
     return 0;
   const int nsx = g.dims[0] + 1, nsy = g.dims[1] + 1, nsz = g.dims[2] + 1;
-  const float w = (float)(1 << g.level);
-  const vec3f pl = mk3((pos.x - lo.x) / w, (pos.y - lo.y) / w, (pos.z - lo.z) / w);
+  // (pos - lo) / 2^level, as an exact scaling: a quotient by a power of two
+  // and the product with its (exact) reciprocal round identically
+  const float rw = ldexpf(1.f, -g.level);
+  const vec3f pl = mk3((pos.x - lo.x) * rw, (pos.y - lo.y) * rw, (pos.z - lo.z) * rw);
   const int ix = (int)pl.x, iy = (int)pl.y, iz = (int)pl.z;
   const int ax = ix + 1 < nsx - 1 ? ix + 1 : nsx - 1;
   const int ay = iy + 1 < nsy - 1 ? iy + 1 : nsy - 1;
@@ -357,11 +359,13 @@ EXA_HD bool intersectNewton(float &value, vec3f P, const float4 *V) {
     pc[0] = pc[0] - d0;
     pc[1] = pc[1] - d1;
     pc[2] = pc[2] - d2;
-    // CONVERGED / DIVERGED are double literals in the reference
-    if (((double)fabsf(d0) < 1e-4) && ((double)fabsf(d1) < 1e-4) && ((double)fabsf(d2) < 1e-4)) {
+    // CONVERGED / DIVERGED are double literals in the reference: for a float
+    // x, (double)x < 1e-4 holds exactly when x < 0x1.a36e30p-14f (the float
+    // just above 1e-4), and 1e6 is a float
+    const float kConverged = 0x1.a36e30p-14f, kDiverged = 1e6f;
+    if (fabsf(d0) < kConverged && fabsf(d1) < kConverged && fabsf(d2) < kConverged) {
       converged = true;
-    } else if (((double)fabsf(pc[0]) > 1e6) || ((double)fabsf(pc[1]) > 1e6) ||
-               ((double)fabsf(pc[2]) > 1e6)) {
+    } else if (fabsf(pc[0]) > kDiverged || fabsf(pc[1]) > kDiverged || fabsf(pc[2]) > kDiverged) {
       return false;
     }
   }

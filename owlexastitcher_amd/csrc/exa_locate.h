@@ -177,6 +177,80 @@ EXA_DEV Sample locateTablesGridlets(const SamplerView &sv, const vec3f p) {
   return s;
 }
 
+// The Woodcock loop's sample lookup with the direct tables: testSample on
+// the cached gridlet (ExaStitchSampler.h:109-126) and, when that misses,
+// full point location (gridlet stage).  Same result as
+// testSampleGridlet() followed by locateTablesGridlets(), but the loads of
+// the cached-gridlet test and of every level's owner tables are issued
+// together, so a lookup costs about four dependent memory round trips.
+// cachedHit tells which of the two answered.
+constexpr int kFastLevels = 8;
+EXA_DEV Sample lookupTablesGridlets(const SamplerView &sv, const vec3f p, int32_t cached, bool &cachedHit) {
+  cachedHit = false;
+  const bool fast = sv.gTop != nullptr && sv.numLevels <= kFastLevels;
+  // round 1: cached gridlet record, owner blocks of every level
+  const Gridlet cg = sv.gridlets[cached >= 0 ? cached : 0];
+  int32_t blk[kFastLevels], local[kFastLevels];
+  // first level at which the owner tables cannot decide (point near a
+  // dual-cell face): the list scan has to take over from there
+  int undecided = kFastLevels;
+#pragma unroll
+  for (int L = 0; L < kFastLevels; ++L) {
+    blk[L] = -1;
+    local[L] = 0;
+    if (!fast || L >= sv.numLevels) continue;
+    const int32_t *D = sv.levelDims[L];
+    const float inv = 1.f / (float)(1 << L);
+    const float gx = p.x * inv - 0.5f, gy = p.y * inv - 0.5f, gz = p.z * inv - 0.5f;
+    const float mxf = floorf(gx * 0.125f);
+    const float myf = floorf(gy * 0.125f);
+    const float mzf = floorf(gz * 0.125f);
+    if (!(mxf >= 0.f && myf >= 0.f && mzf >= 0.f && mxf < (float)D[0] && myf < (float)D[1] &&
+          mzf < (float)D[2]))
+      continue;
+    const float cx = floorf(gx), cy = floorf(gy), cz = floorf(gz);
+    const float fx = gx - cx, fy = gy - cy, fz = gz - cz;
+    const float m0 = 0.0078125f, m1 = 0.9921875f;
+    if (!(fx >= m0 && fx <= m1 && fy >= m0 && fy <= m1 && fz >= m0 && fz <= m1)) {
+      if (undecided == kFastLevels) undecided = L;
+      continue;
+    }
+    const int32_t idx = sv.levelOffset[L] + ((int32_t)mzf * D[1] + (int32_t)myf) * D[0] + (int32_t)mxf;
+    blk[L] = sv.gTop[idx];
+    local[L] = (((int32_t)cz & 7) * 8 + ((int32_t)cy & 7)) * 8 + ((int32_t)cx & 7);
+  }
+  // round 2: owners (the cached test's scalar loads go out with them)
+  int32_t own[kFastLevels];
+#pragma unroll
+  for (int L = 0; L < kFastLevels; ++L)
+    own[L] = blk[L] >= 0 ? sv.gOwner[(int64_t)blk[L] * LocateTables::kOwnerBlock + local[L]] : -1;
+  if (cached >= 0) {
+    float v;
+    int cid;
+    if (intersectGridlet(v, cid, p, cg, sv.gridletScalars)) {
+      cachedHit = true;
+      return Sample{cached, cid, v};
+    }
+  }
+  if (!fast) return locateTablesGridlets(sv, p);
+  // the lowest level whose owner answers, if no undecided level lies below
+  int32_t g = -1;
+#pragma unroll
+  for (int L = kFastLevels - 1; L >= 0; --L)
+    if (own[L] >= 0 && L < undecided) g = own[L];
+  if (g < 0) {
+    if (undecided == kFastLevels) return Sample{-1, -1, 0.f};
+    return locateTablesGridlets(sv, p);
+  }
+  // rounds 3 and 4: owner record and culling flag, then its corner scalars
+  if (!(sv.gridletMaxOp && sv.gridletMaxOp[g] == 0.f)) {
+    float v;
+    int cid;
+    if (intersectGridlet(v, cid, p, sv.gridlets[g], sv.gridletScalars) == 2) return Sample{g, cid, v};
+  }
+  return locateTablesGridlets(sv, p);
+}
+
 // Direct-table point location, element stage: the element list of p's bin,
 // stored AABB first; kind receives 1..4 (tet/pyr/wedge/hex) or -1.
 EXA_DEV Sample locateTablesElements(const SamplerView &sv, const vec3f p, int &kind) {

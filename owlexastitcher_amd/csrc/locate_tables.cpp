@@ -12,6 +12,9 @@
 namespace exa {
 
 // bin index of coordinate x: the same float expression runs on the device
+// device-memory budget of the element candidate records
+static constexpr int64_t kMaxCandidateBytes = 8ll << 30;
+
 static inline int32_t binOf(float x, float lo, float inv, int32_t dims) {
   const float f = floorf((x - lo) * inv);
   int32_t i = f != f ? 0 : (f < 0.f ? 0 : (f > (float)(dims - 1) ? dims - 1 : (int32_t)f));
@@ -127,13 +130,31 @@ void buildLocateTables(const HostModel &m, LocateTables &t) {
     ext[e] = mx;
   }
   std::nth_element(ext.begin(), ext.begin() + ne / 2, ext.end());
+  // Bin size: the power of two at or below half the median element extent
+  // (a few AABB candidates per query), doubled while the bins or their
+  // 32-byte candidate records would outgrow their budget.
   float size = 1.f;
   while (size * 2.f <= ext[ne / 2] && size < 1024.f) size *= 2.f;
+  size *= 0.5f;
   while (size < 1e-3f * std::max(std::max(hi[0] - lo[0], hi[1] - lo[1]), hi[2] - lo[2])) size *= 2.f;
   for (;;) {
     for (int k = 0; k < 3; ++k) t.binDims[k] = (int32_t)floorf((hi[k] - lo[k]) * (1.f / size)) + 1;
-    if ((int64_t)t.binDims[0] * t.binDims[1] * t.binDims[2] <= (1ll << 27)) break;
-    size *= 2.f;
+    if ((int64_t)t.binDims[0] * t.binDims[1] * t.binDims[2] > (1ll << 27)) {
+      size *= 2.f;
+      continue;
+    }
+    int64_t entries = 0;
+    for (int64_t e = 0; e < ne; ++e) {
+      int64_t n = 1;
+      for (int k = 0; k < 3; ++k)
+        n *= binOf(boxes[6 * e + 3 + k], lo[k], 1.f / size, t.binDims[k]) - binOf(boxes[6 * e + k], lo[k], 1.f / size, t.binDims[k]) + 1;
+      entries += n;
+    }
+    if (entries * 32 > kMaxCandidateBytes && size < 1024.f) {
+      size *= 2.f;
+      continue;
+    }
+    break;
   }
   for (int k = 0; k < 3; ++k) t.binLo[k] = lo[k];
   t.binInv = 1.f / size;

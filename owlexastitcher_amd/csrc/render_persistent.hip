@@ -18,7 +18,9 @@
 //  - resolve pass: TF lookup and the Woodcock / ratio-tracking decision.
 // When no lane can make progress in the tracking, gridlet or resolve passes,
 // both gated passes run regardless of their thresholds, so every wave
-// drains.  A lane that finishes its pixel takes the next one from a
+// drains.  Waves never wait for each other (pooling waiting lanes over the
+// workgroup through LDS was tried: the per-round barriers cost more than the
+// fuller element batches gained).  A lane that finishes its pixel takes the next one from a
 // frame-wide work counter (one atomic per wave and pass); pixels are handed
 // out in 8x8 tiles in scan order.
 // Each lane draws random numbers and does its floating-point work in exactly
@@ -250,9 +252,7 @@ EXA_DEV void grabPixels(PathLane &L, const RenderParams &P, const PersistParams 
 
 template <bool kCount, int kAccel>
 __global__ __launch_bounds__(kPersistBlock) __attribute__((amdgpu_waves_per_eu(EXA_PERSIST_MIN_WAVES, 8))) void
-kRenderPersistent(RenderParams P, SamplerView sv, GridView gv,
-                                                                   PersistParams pp,
-                                                                   unsigned long long *counters) {
+kRenderPersistent(RenderParams P, SamplerView sv, GridView gv, PersistParams pp, unsigned long long *counters) {
   __shared__ int32_t stackMem[kAccel == kAccelBvh ? kStackDepth * kPersistBlock : 1];
   int32_t *stack = stackMem + threadIdx.x;
   const int lane = (int)__lane_id();
@@ -289,18 +289,19 @@ kRenderPersistent(RenderParams P, SamplerView sv, GridView gv,
     Sample smp{-1, -1, 0.f};
     bool have = false;
     if (L.st == kStAtSample) {
-      smp = testSampleGridlet(sv, L.pos, L.primID);
+      bool cachedHit;
+      if (kAccel == kAccelTables) {
+        smp = lookupTablesGridlets(sv, L.pos, L.primID, cachedHit);
+      } else {
+        smp = testSampleGridlet(sv, L.pos, L.primID);
+        cachedHit = smp.primID >= 0;
+        if (!cachedHit) smp = locateGridlets<kAccel>(sv, L.pos, stack, kPersistBlock);
+      }
       if (smp.primID >= 0) {
-        if (kCount) cnt.c[kCntCachedHit]++;
+        if (kCount) cnt.c[cachedHit ? kCntCachedHit : kCntGridletHit]++;
         have = true;
       } else {
-        smp = locateGridlets<kAccel>(sv, L.pos, stack, kPersistBlock);
-        if (smp.primID >= 0) {
-          if (kCount) cnt.c[kCntGridletHit]++;
-          have = true;
-        } else {
-          L.st = kStPending;
-        }
+        L.st = kStPending;
       }
     }
 
