@@ -95,51 +95,64 @@ def algorithmic_bytes(c):
 
 def cpu_baseline(r, rgba, dm, args, target_s):
     """The oracle (single-precision restatement of the reference's sampling
-    and integration code) on this host's cores, on a band of rows of the
-    same frame."""
+    and integration code) on this host's cores: whole frames of the same
+    workload, consecutive accumIDs, until about target_s seconds of work."""
+    import ctypes as C
+    from concurrent.futures import ThreadPoolExecutor
     from oracle import oracle as ora
     st = r.state()
     idx, verts, gl, gs = dm.arrays()
-    ng, ne = len(gl), len(idx)
-    gmo, emo = r.sampler.prim_max_opacities(ng, ne)
+    gmo, emo = r.sampler.prim_max_opacities(len(gl), len(idx))
     om = ora.Model(verts, idx, gl, gs, gmo, emo)
     t = time.time()
-    om.bins(64)
+    bins = om.bins(64)
     log(f"[bench] cpu baseline: oracle bins built ({time.time() - t:.1f}s)")
     maj = r.sampler.max_opacities()
     threads = os.cpu_count() or 1
-    rows = 8
-    elapsed = 0.0
-    done_rows = 0
-    y0 = st.H // 2 - 4 * rows
-    while elapsed < target_s and done_rows < st.H:
-        st.H = st.H  # full-frame parameters; only rows [y0, y0+rows) are rendered
-        p = ora.render_params_from_state(st, maj, rgba)
-        accum = np.zeros((st.H, st.W, 4), np.float32)
-        t = time.time()
-        _render_rows(ora, p, om, accum, y0 + done_rows, min(st.H, y0 + done_rows + rows), threads)
-        elapsed += time.time() - t
-        done_rows += rows
-        rows = min(st.H - done_rows, rows * 2)
-        if rows <= 0:
-            break
-    samples = done_rows * st.W * st.spp
-    return {"value": samples / elapsed, "unit": "samples/s", "cores": threads, "kind": "port",
-            "sample": f"{done_rows} rows x {st.W} px x {st.spp} spp of the same 1920x1080 path-traced frame, "
-                      f"oracle f32 restatement on {threads} host threads, {elapsed:.1f}s"}
-
-
-def _render_rows(ora, p, om, accum, y0, y1, threads):
-    import ctypes as C
-    from concurrent.futures import ThreadPoolExecutor
-    fb = np.zeros((p.H, p.W), np.uint32)
-    bins = om.bins()
-
-    def work(y):
-        ora.lib().ora_render_f32(C.byref(p), om.ref, bins, ora.ptr(accum), ora.ptr(fb), y, y + 1)
-
+    accum = np.zeros((st.H, st.W, 4), np.float32)
+    fb = np.zeros((st.H, st.W), np.uint32)
+    frames, elapsed = 0, 0.0
     with ThreadPoolExecutor(threads) as ex:
-        list(ex.map(work, range(y0, y1)))
+        while elapsed < target_s and frames < 1000:
+            st.accumID = frames
+            p = ora.render_params_from_state(st, maj, rgba)
+            t = time.time()
+            list(ex.map(lambda y: ora.lib().ora_render_f32(C.byref(p), om.ref, bins, ora.ptr(accum), ora.ptr(fb),
+                                                           y, y + 1), range(st.H)))
+            elapsed += time.time() - t
+            frames += 1
+    samples = frames * st.H * st.W * st.spp
+    return {"value": samples / elapsed, "unit": "samples/s", "cores": threads, "kind": "port",
+            "sample": f"{frames} whole {st.W}x{st.H} frames ({st.spp} spp, accumIDs 0..{frames - 1}) of the same "
+                      f"workload, oracle f32 restatement, {threads} host threads (one image row per task), "
+                      f"{elapsed:.1f}s"}
+
+
+def copy_calibration(nbytes, dev):
+    """One plain device copy of nbytes (torch's copy kernel): the reference
+    point that scales rocprofv3's FETCH_SIZE / WRITE_SIZE to bytes."""
+    import torch
+    n = nbytes // 4
+    a = torch.ones(n, dtype=torch.float32, device=dev)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    log(f"[bench] copy calibration: {nbytes} bytes read + {nbytes} bytes written")
+    del a, b
+
+
+def read_traffic(args):
+    """HBM bytes per frame of the render kernel from the committed rocprofv3
+    summary for this workload (FETCH_SIZE + WRITE_SIZE, copy-calibrated)."""
+    path = os.path.join(ROOT, "profiles", f"kRender{'Persistent' if args.scheme == 'persistent' else ''}"
+                                          f"_noise3_{args.width}x{args.height}_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_frame"):
+            return float(line.split("=")[1])
+    return None
 
 
 def main():
@@ -157,11 +170,13 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--accel", choices=["tables", "bvh"], default="tables")
     ap.add_argument("--scheme", choices=["persistent", "megakernel"], default="persistent")
-    ap.add_argument("--ctrl-steps", type=int, default=8)
+    ap.add_argument("--ctrl-steps", type=int, default=16)
     ap.add_argument("--elem-batch", type=int, default=16)
-    ap.add_argument("--rare-batch", type=int, default=16)
+    ap.add_argument("--rare-batch", type=int, default=8)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="run one plain device copy of BYTES first (for rocprofv3 counter scaling)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -173,6 +188,8 @@ def main():
     from owlexastitcher_amd import lib
     lib()
     dev = torch.device("cuda:0")
+    if args.calibrate_copy:
+        copy_calibration(args.calibrate_copy, dev)
     amr, dm, model, counts = build_scene(args)
     r, rgba = setup_renderer(model, args)
     stream = torch.cuda.current_stream()
@@ -244,7 +261,7 @@ def main():
             "counters": cnt,
         },
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(args),
                      "kernel": "kRenderPersistent" if args.scheme == "persistent" else "kRender"},
         "cpu_baseline": cpu,
     }

@@ -199,7 +199,7 @@ int exa_renderer_set_accel(exa_renderer *r, int accel);
    Tuning: ctrlSteps = free-flight / macrocell transitions per lane and pass
    (>= 1); elemBatch and rareBatch in [0, 64] = wave fraction (x/64) of live
    lanes that must wait for an element test, resp. for per-segment work,
-   before that pass runs.  Defaults 8, 16, 16. */
+   before that pass runs.  Defaults 16, 16, 8. */
 #define EXA_RENDER_MEGAKERNEL 0
 #define EXA_RENDER_PERSISTENT 1
 int exa_renderer_set_scheme(exa_renderer *r, int scheme);

@@ -124,9 +124,9 @@ void launchRender(const RenderParams &p, const SamplerView &sv, int accel, const
                   unsigned long long *counters, hipStream_t s);
 // scheduling knobs of the persistent renderer (results do not depend on them)
 struct PersistTuning {
-  int32_t ctrlSteps = 8;   // tracking transitions per lane and round, >= 1
+  int32_t ctrlSteps = 16;  // tracking transitions per lane and round, >= 1
   int32_t elemBatch = 16;  // element pass when waiting lanes >= elemBatch/64 of live lanes
-  int32_t rareBatch = 16;  // per-segment pass when waiting lanes >= rareBatch/64 of live lanes
+  int32_t rareBatch = 8;   // per-segment pass when waiting lanes >= rareBatch/64 of live lanes
 };
 // work: one device uint32 (reset by the launch itself)
 void launchRenderPersistent(const RenderParams &p, const SamplerView &sv, int accel, const GridView &gv,

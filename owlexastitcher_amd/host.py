@@ -355,7 +355,7 @@ class OWLRenderer:
             check(rc)
         return rc
 
-    def setPersistentTuning(self, ctrlSteps=8, elemBatch=16, rareBatch=16):
+    def setPersistentTuning(self, ctrlSteps=16, elemBatch=16, rareBatch=8):
         """Scheduling knobs of RENDER_PERSISTENT; images do not depend on them."""
         check(lib().exa_renderer_set_persistent_tuning(self.h, int(ctrlSteps), int(elemBatch), int(rareBatch)))
 
