@@ -230,6 +230,50 @@ typedef struct {
 int exa_renderer_get_state(const exa_renderer *r, exa_render_state *out);
 void exa_renderer_destroy(exa_renderer *r);
 
+/* ---- batched requests (build-side; no reference counterpart) ----
+   Many views, transfer functions and image sizes of uneven size rendered in
+   one persistent launch per frame, sharing the renderer's model, sampler and
+   tables (the reference renders one OWLRenderer view per owlLaunch2D,
+   OWLRenderer.cpp:610-622).  Each request keeps its own accumulation buffer
+   and accumID (seeding as in deviceCode.cu:1278) and gives the image the
+   renderer would give for the same settings.  Lights, clip plane, bounce cap,
+   crosshair and model transforms are taken from the renderer when the
+   request is added. */
+typedef struct exa_batch exa_batch;
+typedef struct {
+  int32_t width, height, spp, integrator; /* integrator: 0 path tracer, 1 direct lighting */
+  int32_t tf;                              /* index from exa_batch_add_transfer_function */
+  float camOrg[3], camDir00[3], camDu[3], camDv[3];
+} exa_batch_request;
+int exa_batch_create(exa_renderer *r, exa_batch **out);
+void exa_batch_destroy(exa_batch *b);
+/* rgba: host float4[numColors]; [lo, hi] = absolute TF domain; opacityScale
+   multiplies extinction directly (the renderer's setOpacityScale slider value
+   s corresponds to 1.1^(s-100)). */
+int exa_batch_add_transfer_function(exa_batch *b, const float *rgba, int numColors, float lo, float hi,
+                                    float opacityScale, int *index);
+int exa_batch_add_request(exa_batch *b, const exa_batch_request *req, int *index);
+int exa_batch_num_requests(const exa_batch *b);
+int exa_batch_set_tuning(exa_batch *b, int ctrlSteps, int elemBatch, int rareBatch);
+int exa_batch_get_tuning(const exa_batch *b, int tuning[3]);
+/* one frame of every request (asynchronous on stream); accumIDs advance */
+int exa_batch_render(exa_batch *b, void *stream);
+int exa_batch_render_counted(exa_batch *b, uint64_t counters[10], void *stream);
+/* Captures one frame of the batch into a HIP graph on a non-default stream;
+   exa_batch_render_graph replays it (the accumIDs live on the device and
+   advance inside the graph).  Any change to the batch drops the graph. */
+int exa_batch_capture_graph(exa_batch *b, void *stream);
+int exa_batch_render_graph(exa_batch *b, void *stream);
+int exa_batch_reset_accum(exa_batch *b);
+int exa_batch_get_accum_id(const exa_batch *b, int req);
+int exa_batch_get_accum(const exa_batch *b, int req, float *hostOut);     /* float4 per pixel */
+int exa_batch_get_frame(const exa_batch *b, int req, uint32_t *hostOut);  /* RGBA8 per pixel */
+/* Autotuner: one timed frame per combination of candidate knob values;
+   keeps the fastest, returns it in best[3] (and its time), resets the
+   accumulation. */
+int exa_batch_autotune(exa_batch *b, const int *ctrlSteps, int nCtrl, const int *elemBatch, int nElem,
+                       const int *rareBatch, int nRare, void *stream, int best[3], float *bestMs);
+
 /* ---- host helpers ---- */
 /* Camera from eye/point-of-interest/up/fovy for a W x H image
    (viewer.cpp:79-107 SimpleCamera, 550-560 cameraChanged). */

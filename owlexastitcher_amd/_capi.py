@@ -46,6 +46,15 @@ class StitchModelDesc(C.Structure):
     ]
 
 
+class BatchRequestDesc(C.Structure):
+    _fields_ = [
+        ("width", C.c_int32), ("height", C.c_int32), ("spp", C.c_int32), ("integrator", C.c_int32),
+        ("tf", C.c_int32),
+        ("camOrg", C.c_float * 3), ("camDir00", C.c_float * 3), ("camDu", C.c_float * 3),
+        ("camDv", C.c_float * 3),
+    ]
+
+
 class RenderState(C.Structure):
     _fields_ = [
         ("W", C.c_int32), ("H", C.c_int32), ("spp", C.c_int32), ("accumID", C.c_int32),
@@ -138,6 +147,22 @@ _SIGS = {
     "exa_renderer_set_persistent_tuning": (C.c_int, [P, C.c_int, C.c_int, C.c_int]),
     "exa_renderer_get_state": (C.c_int, [P, P]),
     "exa_renderer_destroy": (None, [P]),
+    "exa_batch_create": (C.c_int, [P, P]),
+    "exa_batch_destroy": (None, [P]),
+    "exa_batch_add_transfer_function": (C.c_int, [P, P, C.c_int, F, F, F, P]),
+    "exa_batch_add_request": (C.c_int, [P, P, P]),
+    "exa_batch_num_requests": (C.c_int, [P]),
+    "exa_batch_set_tuning": (C.c_int, [P, C.c_int, C.c_int, C.c_int]),
+    "exa_batch_get_tuning": (C.c_int, [P, P]),
+    "exa_batch_render": (C.c_int, [P, P]),
+    "exa_batch_render_counted": (C.c_int, [P, P, P]),
+    "exa_batch_capture_graph": (C.c_int, [P, P]),
+    "exa_batch_render_graph": (C.c_int, [P, P]),
+    "exa_batch_reset_accum": (C.c_int, [P]),
+    "exa_batch_get_accum_id": (C.c_int, [P, C.c_int]),
+    "exa_batch_get_accum": (C.c_int, [P, C.c_int, P]),
+    "exa_batch_get_frame": (C.c_int, [P, C.c_int, P]),
+    "exa_batch_autotune": (C.c_int, [P, P, C.c_int, P, C.c_int, P, C.c_int, P, P, P]),
     "exa_camera_look_at": (C.c_int, [P, P, P, F, C.c_int, C.c_int, P, P, P, P]),
     "exa_xf_load": (C.c_int, [C.c_char_p, P, P, P, P, P, C.c_int]),
     "exa_tf_rasterize": (C.c_int, [P, C.c_int, C.c_int, P]),

@@ -361,6 +361,67 @@ struct exa_renderer {
   }
 };
 
+// A batch of render requests sharing one renderer's model, sampler and
+// tables; all of them run in one persistent launch per frame.
+struct exa_batch {
+  exa_renderer *r = nullptr;
+  struct Tf {
+    DeviceBuffer<float4> colorMap;
+    DeviceBuffer<float> majorants, gridletMaxOp, elemMaxOp;
+    float lo = 0.f, hi = 1.f, opacityScale = 1.f;
+    int n = 0;
+  };
+  struct Req {
+    int tf = 0;
+    DeviceBuffer<float4> accum;
+    DeviceBuffer<uint32_t> fb;
+  };
+  std::vector<std::unique_ptr<Tf>> tfs;
+  std::vector<std::unique_ptr<Req>> reqs;
+  std::vector<BatchRequest> host;  // mirror of the device array (accumIDs advanced on both)
+  DeviceBuffer<BatchRequest> dev;
+  DeviceBuffer<uint32_t> work;
+  uint32_t numWork = 0;
+  bool dirty = true;
+  PersistTuning tuning;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+
+  ~exa_batch() { dropGraph(); }
+  void dropGraph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+  }
+  // uploads the request array when requests changed (synchronous)
+  void sync() {
+    if (!dirty) return;
+    if (host.empty()) throw Status(EXA_ERR_STATE, "batch has no requests");
+    uint32_t w = 0;
+    for (BatchRequest &b : host) {
+      b.workBegin = w;
+      const uint64_t tiles = (uint64_t)b.tilesX * (uint64_t)((b.P.H + 7) / 8);
+      if ((uint64_t)w + tiles * 64 >= (1ull << 32)) throw Status(EXA_ERR_INVALID, "batch too large");
+      w += (uint32_t)(tiles * 64);
+    }
+    numWork = w;
+    dropGraph();
+    dev.alloc(host.size());
+    HIP_CHECK(hipDeviceSynchronize());
+    HIP_CHECK(hipMemcpy(dev.ptr, host.data(), host.size() * sizeof(BatchRequest), hipMemcpyHostToDevice));
+    work.alloc(1);
+    dirty = false;
+  }
+  void launch(unsigned long long *counters, hipStream_t s) {
+    launchRenderBatch(r->sampler.view(), r->sampler.accel, r->sampler.gridView(), dev.ptr, (int)host.size(),
+                      numWork, work.ptr, tuning, counters, s);
+  }
+  void advanced() {
+    for (BatchRequest &b : host) b.P.accumID += 1;
+  }
+};
+
 // ---------------------------------------------------------------------------
 extern "C" {
 
@@ -919,6 +980,245 @@ int exa_renderer_set_persistent_tuning(exa_renderer *r, int ctrlSteps, int elemB
     r->tuning.ctrlSteps = ctrlSteps;
     r->tuning.elemBatch = elemBatch;
     r->tuning.rareBatch = rareBatch;
+  });
+}
+// ---- batched requests -------------------------------------------------------
+int exa_batch_create(exa_renderer *r, exa_batch **out) {
+  return guarded([&] {
+    require(r && out, "invalid arguments");
+    auto b = std::make_unique<exa_batch>();
+    b->r = r;
+    *out = b.release();
+  });
+}
+void exa_batch_destroy(exa_batch *b) { delete b; }
+int exa_batch_add_transfer_function(exa_batch *b, const float *rgba, int numColors, float lo, float hi,
+                                    float opacityScale, int *index) {
+  return guarded([&] {
+    require(b && rgba && numColors > 0 && index, "invalid arguments");
+    require(b->tfs.size() < 256, "too many transfer functions");
+    exa_renderer *r = b->r;
+    auto t = std::make_unique<exa_batch::Tf>();
+    t->colorMap.upload((const float4 *)rgba, numColors, 0);
+    t->lo = lo;
+    t->hi = hi;
+    t->opacityScale = opacityScale;
+    t->n = numColors;
+    // majorants and culling of this TF, computed by the shared sampler, then
+    // the renderer's own state is put back
+    exa_sampler &S = r->sampler;
+    const bool hadMaxOp = S.haveMaxOp;
+    S.computeMaxOpacities(rgba, numColors, lo, hi, 0);
+    t->majorants.alloc(S.maxOpacities.count);
+    t->gridletMaxOp.alloc(S.gridletMaxOp.count);
+    t->elemMaxOp.alloc(S.elemMaxOp.count);
+    if (S.maxOpacities.count)
+      HIP_CHECK(hipMemcpy(t->majorants.ptr, S.maxOpacities.ptr, S.maxOpacities.bytes(), hipMemcpyDeviceToDevice));
+    if (S.gridletMaxOp.count)
+      HIP_CHECK(hipMemcpy(t->gridletMaxOp.ptr, S.gridletMaxOp.ptr, S.gridletMaxOp.bytes(), hipMemcpyDeviceToDevice));
+    if (S.elemMaxOp.count)
+      HIP_CHECK(hipMemcpy(t->elemMaxOp.ptr, S.elemMaxOp.ptr, S.elemMaxOp.bytes(), hipMemcpyDeviceToDevice));
+    if (!r->colorMap.empty()) r->updateMaxOpacities();
+    else S.haveMaxOp = hadMaxOp;
+    *index = (int)b->tfs.size();
+    b->tfs.push_back(std::move(t));
+  });
+}
+int exa_batch_add_request(exa_batch *b, const exa_batch_request *q, int *index) {
+  return guarded([&] {
+    require(b && q && index, "invalid arguments");
+    require(q->width > 0 && q->height > 0 && (int64_t)q->width * q->height < (1ll << 31), "invalid image size");
+    require(q->spp >= 1, "spp must be >= 1");
+    require(q->integrator == 0 || q->integrator == 1, "supported integrators: 0 path tracer, 1 direct lighting");
+    require(q->tf >= 0 && q->tf < (int)b->tfs.size(), "unknown transfer function");
+    require((int)b->host.size() < kMaxBatchRequests, "too many requests in one batch");
+    const exa_batch::Tf &t = *b->tfs[q->tf];
+    auto rq = std::make_unique<exa_batch::Req>();
+    rq->tf = q->tf;
+    rq->accum.alloc((size_t)q->width * q->height);
+    rq->fb.alloc((size_t)q->width * q->height);
+    BatchRequest br;
+    memset(&br, 0, sizeof(br));
+    br.P = b->r->P;  // model transforms, lights, clip plane, bounce cap, crosshair
+    RenderParams &P = br.P;
+    P.W = q->width;
+    P.H = q->height;
+    P.spp = q->spp;
+    P.accumID = 0;
+    P.integrator = q->integrator;
+    memcpy(P.camOrg, q->camOrg, 12);
+    memcpy(P.camDir00, q->camDir00, 12);
+    memcpy(P.camDu, q->camDu, 12);
+    memcpy(P.camDv, q->camDv, 12);
+    P.tfDomain[0] = t.lo;
+    P.tfDomain[1] = t.hi;
+    P.opacityScale = t.opacityScale;
+    P.tf = t.colorMap.ptr;
+    P.tfN = t.n;
+    P.subImageActive = 0;
+    P.subImageSelecting = 0;
+    P.accum = rq->accum.ptr;
+    P.fb = rq->fb.ptr;
+    br.majorants = t.majorants.ptr;
+    br.gridletMaxOp = t.gridletMaxOp.ptr;
+    br.elemMaxOp = t.elemMaxOp.ptr;
+    int nl = 0;
+    for (int i = 0; i < kLightsMax; ++i) nl += P.lights[i].on ? 1 : 0;
+    br.numLights = nl;
+    br.tilesX = (q->width + 7) / 8;
+    *index = (int)b->host.size();
+    b->host.push_back(br);
+    b->reqs.push_back(std::move(rq));
+    b->dirty = true;
+  });
+}
+int exa_batch_num_requests(const exa_batch *b) { return b ? (int)b->host.size() : EXA_ERR_INVALID; }
+int exa_batch_set_tuning(exa_batch *b, int ctrlSteps, int elemBatch, int rareBatch) {
+  return guarded([&] {
+    require(b, "batch is NULL");
+    require(ctrlSteps >= 1 && ctrlSteps <= 1024 && elemBatch >= 0 && elemBatch <= 64 && rareBatch >= 0 &&
+                rareBatch <= 64,
+            "tuning out of range");
+    b->tuning.ctrlSteps = ctrlSteps;
+    b->tuning.elemBatch = elemBatch;
+    b->tuning.rareBatch = rareBatch;
+    b->dropGraph();
+  });
+}
+int exa_batch_get_tuning(const exa_batch *b, int tuning[3]) {
+  if (!b || !tuning) return EXA_ERR_INVALID;
+  tuning[0] = b->tuning.ctrlSteps;
+  tuning[1] = b->tuning.elemBatch;
+  tuning[2] = b->tuning.rareBatch;
+  return EXA_OK;
+}
+int exa_batch_render(exa_batch *b, void *stream) {
+  return guarded([&] {
+    require(b, "batch is NULL");
+    if (b->r->tf.count == 0 && b->tfs.empty()) throw Status(EXA_ERR_STATE, "no transfer function");
+    b->sync();
+    b->launch(nullptr, (hipStream_t)stream);
+    HIP_CHECK(hipGetLastError());
+    b->advanced();
+  });
+}
+int exa_batch_render_counted(exa_batch *b, uint64_t counters[10], void *stream) {
+  return guarded([&] {
+    require(b && counters, "invalid arguments");
+    b->sync();
+    hipStream_t st = (hipStream_t)stream;
+    unsigned long long *dc = b->r->sampler.counters.ptr;
+    HIP_CHECK(hipMemsetAsync(dc, 0, kNumCounters * sizeof(unsigned long long), st));
+    b->launch(dc, st);
+    HIP_CHECK(hipGetLastError());
+    b->advanced();
+    unsigned long long h[kNumCounters];
+    HIP_CHECK(hipMemcpyAsync(h, dc, sizeof(h), hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    for (int i = 0; i < kNumCounters; ++i) counters[i] = h[i];
+  });
+}
+int exa_batch_capture_graph(exa_batch *b, void *stream) {
+  return guarded([&] {
+    require(b && stream, "a non-default stream is needed for capture");
+    b->sync();
+    b->dropGraph();
+    hipStream_t st = (hipStream_t)stream;
+    HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    b->launch(nullptr, st);
+    hipGraph_t g = nullptr;
+    const hipError_t e1 = hipGetLastError();
+    const hipError_t e2 = hipStreamEndCapture(st, &g);
+    if (e1 != hipSuccess || e2 != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      throw Status(EXA_ERR_HIP, std::string("graph capture failed: ") + hipGetErrorString(e1 != hipSuccess ? e1 : e2));
+    }
+    b->graph = g;
+    HIP_CHECK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  });
+}
+int exa_batch_render_graph(exa_batch *b, void *stream) {
+  return guarded([&] {
+    require(b, "batch is NULL");
+    if (!b->exec || b->dirty) throw Status(EXA_ERR_STATE, "no captured graph (capture after the last change)");
+    HIP_CHECK(hipGraphLaunch(b->exec, (hipStream_t)stream));
+    b->advanced();
+  });
+}
+int exa_batch_reset_accum(exa_batch *b) {
+  return guarded([&] {
+    require(b, "batch is NULL");
+    for (BatchRequest &q : b->host) q.P.accumID = 0;
+    b->dirty = true;
+  });
+}
+int exa_batch_get_accum_id(const exa_batch *b, int req) {
+  if (!b || req < 0 || req >= (int)b->host.size()) return EXA_ERR_INVALID;
+  return b->host[req].P.accumID;
+}
+int exa_batch_get_accum(const exa_batch *b, int req, float *out) {
+  return guarded([&] {
+    require(b && out && req >= 0 && req < (int)b->reqs.size(), "invalid arguments");
+    download(b->reqs[req]->accum, out);
+  });
+}
+int exa_batch_get_frame(const exa_batch *b, int req, uint32_t *out) {
+  return guarded([&] {
+    require(b && out && req >= 0 && req < (int)b->reqs.size(), "invalid arguments");
+    download(b->reqs[req]->fb, out);
+  });
+}
+// Tries every combination of the candidate knob values on one frame each
+// (timed with HIP events on `stream`), keeps the fastest and resets the
+// accumulation (the trial frames accumulated).
+int exa_batch_autotune(exa_batch *b, const int *ctrlSteps, int nCtrl, const int *elemBatch, int nElem,
+                       const int *rareBatch, int nRare, void *stream, int best[3], float *bestMs) {
+  return guarded([&] {
+    require(b && ctrlSteps && elemBatch && rareBatch && nCtrl > 0 && nElem > 0 && nRare > 0 && best,
+            "invalid arguments");
+    b->sync();
+    hipStream_t st = (hipStream_t)stream;
+    hipEvent_t e0, e1;
+    HIP_CHECK(hipEventCreate(&e0));
+    HIP_CHECK(hipEventCreate(&e1));
+    const PersistTuning saved = b->tuning;
+    float bestT = 3.4e38f;
+    PersistTuning bestK = saved;
+    b->launch(nullptr, st);  // warm-up
+    b->advanced();
+    for (int i = 0; i < nCtrl; ++i)
+      for (int j = 0; j < nElem; ++j)
+        for (int k = 0; k < nRare; ++k) {
+          PersistTuning t;
+          t.ctrlSteps = ctrlSteps[i];
+          t.elemBatch = elemBatch[j];
+          t.rareBatch = rareBatch[k];
+          require(t.ctrlSteps >= 1 && t.elemBatch >= 0 && t.elemBatch <= 64 && t.rareBatch >= 0 &&
+                      t.rareBatch <= 64,
+                  "tuning candidate out of range");
+          b->tuning = t;
+          HIP_CHECK(hipEventRecord(e0, st));
+          b->launch(nullptr, st);
+          HIP_CHECK(hipEventRecord(e1, st));
+          HIP_CHECK(hipEventSynchronize(e1));
+          b->advanced();
+          float ms = 0.f;
+          HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+          if (ms < bestT) {
+            bestT = ms;
+            bestK = t;
+          }
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    b->tuning = bestK;
+    b->dropGraph();
+    for (BatchRequest &q : b->host) q.P.accumID = 0;
+    b->dirty = true;
+    best[0] = bestK.ctrlSteps;
+    best[1] = bestK.elemBatch;
+    best[2] = bestK.rareBatch;
+    if (bestMs) *bestMs = bestT;
   });
 }
 int exa_renderer_get_state(const exa_renderer *r, exa_render_state *o) {

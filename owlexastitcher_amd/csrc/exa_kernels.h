@@ -122,6 +122,22 @@ struct RenderParams {
 
 void launchRender(const RenderParams &p, const SamplerView &sv, int accel, const GridView &gv,
                   unsigned long long *counters, hipStream_t s);
+// One request of a batched launch: its own camera, image, spp, integrator,
+// transfer function (and the majorants and culling that follow from it) and
+// accumulation / frame buffers; the model and point-location tables are
+// shared by all requests.
+struct BatchRequest {
+  RenderParams P;
+  const float *majorants;
+  const float *gridletMaxOp;
+  const float *elemMaxOp;
+  int32_t numLights;
+  int32_t tilesX;     // 8x8 pixel tiles per row
+  uint32_t workBegin;  // first work id (64 per tile); ascending over requests
+  uint32_t pad;
+};
+constexpr int kMaxBatchRequests = 1024;
+
 // scheduling knobs of the persistent renderer (results do not depend on them)
 struct PersistTuning {
   int32_t ctrlSteps = 16;  // tracking transitions per lane and round, >= 1
@@ -132,5 +148,10 @@ struct PersistTuning {
 void launchRenderPersistent(const RenderParams &p, const SamplerView &sv, int accel, const GridView &gv,
                             uint32_t *work, const PersistTuning &t, unsigned long long *counters,
                             hipStream_t s);
+// Batched launch over reqs[0..numReqs) (device array); numWork = total work
+// ids.  Afterwards every request's accumID is advanced by one on the device.
+void launchRenderBatch(const SamplerView &sv, int accel, const GridView &gv, BatchRequest *reqs, int numReqs,
+                       uint32_t numWork, uint32_t *work, const PersistTuning &t, unsigned long long *counters,
+                       hipStream_t s);
 
 }  // namespace exa

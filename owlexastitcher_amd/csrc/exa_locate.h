@@ -12,6 +12,15 @@
 
 namespace exa {
 
+// The per-primitive max-opacity arrays a lookup culls with (nullptr: none).
+// They follow from the transfer function, so a batched launch passes each
+// lane its own.
+struct Culling {
+  const float *gridletMaxOp;
+  const float *elemMaxOp;
+};
+EXA_DEV Culling cullingOf(const SamplerView &sv) { return Culling{sv.gridletMaxOp, sv.elemMaxOp}; }
+
 // Binary-BVH point query with a per-thread stack in LDS (stack[i*stride]).
 // leaf(first, count) returns true to end the traversal.  The build caps the
 // depth at kStackDepth, and a DFS stack holds at most one entry per level.
@@ -115,7 +124,7 @@ EXA_DEV int testElementRecord(float &value, vec3f P, int nv, const float4 *__res
 // gridlet cell ends the search (dual cells of one mesh tile space, so no
 // lower-ID gridlet can also hold the point); a boundary hit only bounds the
 // IDs still worth testing.
-EXA_DEV Sample locateTablesGridlets(const SamplerView &sv, const vec3f p) {
+EXA_DEV Sample locateTablesGridlets(const SamplerView &sv, const Culling &cu, const vec3f p) {
   Sample s{-1, -1, 0.f};
   int32_t best = 0x7fffffff;
   for (int L = 0; L < sv.numLevels; ++L) {
@@ -143,7 +152,7 @@ EXA_DEV Sample locateTablesGridlets(const SamplerView &sv, const vec3f p) {
         const int32_t local = (((int32_t)cz & 7) * 8 + ((int32_t)cy & 7)) * 8 + ((int32_t)cx & 7);
         const int32_t g = sv.gOwner[(int64_t)blk * LocateTables::kOwnerBlock + local];
         if (g < 0) continue;
-        if (!(sv.gridletMaxOp && sv.gridletMaxOp[g] == 0.f)) {
+        if (!(cu.gridletMaxOp && cu.gridletMaxOp[g] == 0.f)) {
           float v;
           int cid;
           if (intersectGridlet(v, cid, p, sv.gridlets[g], sv.gridletScalars) == 2) {
@@ -160,7 +169,7 @@ EXA_DEV Sample locateTablesGridlets(const SamplerView &sv, const vec3f p) {
     for (uint32_t k = sv.gStart[idx]; k < k1; ++k) {
       const int32_t g = sv.gList[k];
       if (g >= best) break;
-      if (sv.gridletMaxOp && sv.gridletMaxOp[g] == 0.f) continue;
+      if (cu.gridletMaxOp && cu.gridletMaxOp[g] == 0.f) continue;
       float v;
       int cid;
       const int r = intersectGridlet(v, cid, p, sv.gridlets[g], sv.gridletScalars);
@@ -185,7 +194,8 @@ EXA_DEV Sample locateTablesGridlets(const SamplerView &sv, const vec3f p) {
 // together, so a lookup costs about four dependent memory round trips.
 // cachedHit tells which of the two answered.
 constexpr int kFastLevels = 8;
-EXA_DEV Sample lookupTablesGridlets(const SamplerView &sv, const vec3f p, int32_t cached, bool &cachedHit) {
+EXA_DEV Sample lookupTablesGridlets(const SamplerView &sv, const Culling &cu, const vec3f p, int32_t cached,
+                                    bool &cachedHit) {
   cachedHit = false;
   const bool fast = sv.gTop != nullptr && sv.numLevels <= kFastLevels;
   // round 1: cached gridlet record, owner blocks of every level
@@ -232,7 +242,7 @@ EXA_DEV Sample lookupTablesGridlets(const SamplerView &sv, const vec3f p, int32_
       return Sample{cached, cid, v};
     }
   }
-  if (!fast) return locateTablesGridlets(sv, p);
+  if (!fast) return locateTablesGridlets(sv, cu, p);
   // the lowest level whose owner answers, if no undecided level lies below
   int32_t g = -1;
 #pragma unroll
@@ -240,20 +250,20 @@ EXA_DEV Sample lookupTablesGridlets(const SamplerView &sv, const vec3f p, int32_
     if (own[L] >= 0 && L < undecided) g = own[L];
   if (g < 0) {
     if (undecided == kFastLevels) return Sample{-1, -1, 0.f};
-    return locateTablesGridlets(sv, p);
+    return locateTablesGridlets(sv, cu, p);
   }
   // rounds 3 and 4: owner record and culling flag, then its corner scalars
-  if (!(sv.gridletMaxOp && sv.gridletMaxOp[g] == 0.f)) {
+  if (!(cu.gridletMaxOp && cu.gridletMaxOp[g] == 0.f)) {
     float v;
     int cid;
     if (intersectGridlet(v, cid, p, sv.gridlets[g], sv.gridletScalars) == 2) return Sample{g, cid, v};
   }
-  return locateTablesGridlets(sv, p);
+  return locateTablesGridlets(sv, cu, p);
 }
 
 // Direct-table point location, element stage: the element list of p's bin,
 // stored AABB first; kind receives 1..4 (tet/pyr/wedge/hex) or -1.
-EXA_DEV Sample locateTablesElements(const SamplerView &sv, const vec3f p, int &kind) {
+EXA_DEV Sample locateTablesElements(const SamplerView &sv, const Culling &cu, const vec3f p, int &kind) {
   Sample s{-1, -1, 0.f};
   kind = -1;
   if (sv.numElements <= 0) return s;
@@ -282,7 +292,7 @@ EXA_DEV Sample locateTablesElements(const SamplerView &sv, const vec3f p, int &k
             p.z <= hi[j].z))
         continue;
       const int32_t e = __float_as_int(lo[j].w);
-      if (sv.elemMaxOp && sv.elemMaxOp[e] == 0.f) continue;
+      if (cu.elemMaxOp && cu.elemMaxOp[e] == 0.f) continue;
       float v;
       const int t = testElementRecord(v, p, __float_as_int(hi[j].w), sv.eVerts + 8 * (int64_t)e);
       if (t >= 0) {
@@ -298,7 +308,8 @@ EXA_DEV Sample locateTablesElements(const SamplerView &sv, const vec3f p, int &k
 }
 
 // BVH point location, gridlet stage.
-EXA_DEV Sample locateBvhGridlets(const SamplerView &sv, const vec3f p, int32_t *stack, const int stride) {
+EXA_DEV Sample locateBvhGridlets(const SamplerView &sv, const Culling &cu, const vec3f p, int32_t *stack,
+                                 const int stride) {
   Sample s{-1, -1, 0.f};
   if (sv.numGridlets <= 0) return s;
   int32_t best = 0x7fffffff;
@@ -306,7 +317,7 @@ EXA_DEV Sample locateBvhGridlets(const SamplerView &sv, const vec3f p, int32_t *
     for (int32_t k = 0; k < count; ++k) {
       const int32_t g = sv.gRefs[first + k];
       if (g >= best) break;  // refs ascend within a leaf
-      if (sv.gridletMaxOp && sv.gridletMaxOp[g] == 0.f) continue;
+      if (cu.gridletMaxOp && cu.gridletMaxOp[g] == 0.f) continue;
       float v;
       int cid;
       const int r = intersectGridlet(v, cid, p, sv.gridlets[g], sv.gridletScalars);
@@ -325,7 +336,8 @@ EXA_DEV Sample locateBvhGridlets(const SamplerView &sv, const vec3f p, int32_t *
 }
 
 // BVH point location, element stage; kind receives 1..4 or -1.
-EXA_DEV Sample locateBvhElements(const SamplerView &sv, const vec3f p, int32_t *stack, const int stride,
+EXA_DEV Sample locateBvhElements(const SamplerView &sv, const Culling &cu, const vec3f p, int32_t *stack,
+                                 const int stride,
                                  int &kind) {
   Sample s{-1, -1, 0.f};
   kind = -1;
@@ -336,7 +348,7 @@ EXA_DEV Sample locateBvhElements(const SamplerView &sv, const vec3f p, int32_t *
     for (int32_t k = 0; k < count; ++k) {
       const int32_t e = sv.eRefs[first + k];
       if (e >= best) break;
-      if (sv.elemMaxOp && sv.elemMaxOp[e] == 0.f) continue;
+      if (cu.elemMaxOp && cu.elemMaxOp[e] == 0.f) continue;
       float v;
       const int t = testElement(v, p, sv.indices + 8 * (int64_t)e, sv.vertices);
       if (t >= 0) {
@@ -356,25 +368,28 @@ EXA_DEV Sample locateBvhElements(const SamplerView &sv, const vec3f p, int32_t *
 
 // The two stages of point location, by accelerator.
 template <int kAccel>
-EXA_DEV Sample locateGridlets(const SamplerView &sv, const vec3f p, int32_t *stack, const int stride) {
-  if (kAccel == kAccelTables) return locateTablesGridlets(sv, p);
-  return locateBvhGridlets(sv, p, stack, stride);
+EXA_DEV Sample locateGridlets(const SamplerView &sv, const Culling &cu, const vec3f p, int32_t *stack,
+                              const int stride) {
+  if (kAccel == kAccelTables) return locateTablesGridlets(sv, cu, p);
+  return locateBvhGridlets(sv, cu, p, stack, stride);
 }
 template <int kAccel>
-EXA_DEV Sample locateElements(const SamplerView &sv, const vec3f p, int32_t *stack, const int stride, int &kind) {
-  if (kAccel == kAccelTables) return locateTablesElements(sv, p, kind);
-  return locateBvhElements(sv, p, stack, stride, kind);
+EXA_DEV Sample locateElements(const SamplerView &sv, const Culling &cu, const vec3f p, int32_t *stack,
+                              const int stride, int &kind) {
+  if (kAccel == kAccelTables) return locateTablesElements(sv, cu, p, kind);
+  return locateBvhElements(sv, cu, p, stack, stride, kind);
 }
 
 // Full point location; kind receives -1 miss, 0 gridlet, 1..4 tet/pyr/wedge/hex.
 template <int kAccel>
 EXA_DEV Sample locatePoint(const SamplerView &sv, const vec3f p, int32_t *stack, const int stride, int &kind) {
-  const Sample s = locateGridlets<kAccel>(sv, p, stack, stride);
+  const Culling cu = cullingOf(sv);
+  const Sample s = locateGridlets<kAccel>(sv, cu, p, stack, stride);
   if (s.primID >= 0) {
     kind = 0;
     return s;
   }
-  return locateElements<kAccel>(sv, p, stack, stride, kind);
+  return locateElements<kAccel>(sv, cu, p, stack, stride, kind);
 }
 
 // sampler/ExaStitchSampler.h:109-126 testSample

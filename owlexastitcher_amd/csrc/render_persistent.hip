@@ -61,6 +61,7 @@ EXA_DEV bool isRare(int st) { return st >= kStSegEnd && st < kStDone; }
 
 struct PathLane {
   int32_t st, kind, px, py, s;
+  int32_t req;  // request of a batched launch
   Lcg rng;
   float4 acc;
   // integrator state (pathTracingIntegrate / directLightingIntegrate); the
@@ -81,7 +82,9 @@ struct PathLane {
 
 struct PersistParams {
   uint32_t *work;     // frame-wide pixel-tile counter, zero at launch
-  uint32_t numWork;   // tilesX * tilesY * 64
+  uint32_t numWork;   // 64 per 8x8 pixel tile (of all requests)
+  const BatchRequest *reqs;  // batched launch: the requests
+  int32_t numReqs;
   int32_t tilesX;
   int32_t ctrlSteps;  // tracking transitions per lane and round
   int32_t elemBatch;  // element pass runs when pending*64 >= live*elemBatch
@@ -224,8 +227,12 @@ EXA_DEV void sampleStart(PathLane &L, const RenderParams &P, const GridView &gv,
   }
 }
 
-// hand out pixels to the lanes that wait for one: one atomic per wave
-EXA_DEV void grabPixels(PathLane &L, const RenderParams &P, const PersistParams &pp, int lane) {
+// hand out pixels to the lanes that wait for one: one atomic per wave.  A
+// batched launch numbers the tiles of all requests consecutively
+// (workBegin); a lane finds its request by binary search in LDS.
+template <bool kBatch>
+EXA_DEV void grabPixels(PathLane &L, const RenderParams &P, const PersistParams &pp, const uint32_t *workBegin,
+                        int lane) {
   const uint64_t want = __ballot(L.st == kStGrab);
   if (!want) return;
   const int leader = __ffsll((unsigned long long)want) - 1;
@@ -233,56 +240,96 @@ EXA_DEV void grabPixels(PathLane &L, const RenderParams &P, const PersistParams 
   if (lane == leader) base = atomicAdd(pp.work, (uint32_t)__popcll(want));
   base = __shfl(base, leader);
   if (L.st != kStGrab) return;
-  const uint32_t w = base + (uint32_t)__popcll(want & ((1ull << lane) - 1ull));
+  uint32_t w = base + (uint32_t)__popcll(want & ((1ull << lane) - 1ull));
   if (w >= pp.numWork) {
     L.st = kStDone;
     return;
   }
+  int32_t tilesX = (P.W + kPixelTile - 1) / kPixelTile, W = P.W, H = P.H, accumID = P.accumID;
+  if (kBatch) {
+    int lo = 0, hi = pp.numReqs - 1;  // last request with workBegin <= w
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (workBegin[mid] <= w) lo = mid;
+      else hi = mid - 1;
+    }
+    L.req = lo;
+    w -= workBegin[lo];
+    const BatchRequest &R = pp.reqs[lo];
+    tilesX = R.tilesX;
+    W = R.P.W;
+    H = R.P.H;
+    accumID = R.P.accumID;
+  }
   const uint32_t tile = w >> 6, in = w & 63u;
-  const int px = (int)(tile % (uint32_t)pp.tilesX) * kPixelTile + (int)(in & 7u);
-  const int py = (int)(tile / (uint32_t)pp.tilesX) * kPixelTile + (int)(in >> 3);
-  if (px >= P.W || py >= P.H) return;  // outside a partial edge tile: grab again
+  const int px = (int)(tile % (uint32_t)tilesX) * kPixelTile + (int)(in & 7u);
+  const int py = (int)(tile / (uint32_t)tilesX) * kPixelTile + (int)(in >> 3);
+  if (px >= W || py >= H) return;  // outside a partial edge tile: grab again
   L.px = px;
   L.py = py;
-  L.rng.init((uint32_t)(px + P.W * py), (uint32_t)P.accumID);
+  L.rng.init((uint32_t)(px + W * py), (uint32_t)accumID);
   L.acc = make_float4(0.f, 0.f, 0.f, 0.f);
   L.s = 0;
   L.st = kStSampleStart;
 }
 
-template <bool kCount, int kAccel>
+template <bool kCount, int kAccel, bool kBatch>
 __global__ __launch_bounds__(kPersistBlock) __attribute__((amdgpu_waves_per_eu(EXA_PERSIST_MIN_WAVES, 8))) void
 kRenderPersistent(RenderParams P, SamplerView sv, GridView gv, PersistParams pp, unsigned long long *counters) {
   __shared__ int32_t stackMem[kAccel == kAccelBvh ? kStackDepth * kPersistBlock : 1];
+  __shared__ uint32_t workBegin[kBatch ? kMaxBatchRequests : 1];
   int32_t *stack = stackMem + threadIdx.x;
   const int lane = (int)__lane_id();
+  if (kBatch) {
+    for (int i = threadIdx.x; i < pp.numReqs; i += kPersistBlock) workBegin[i] = pp.reqs[i].workBegin;
+    __syncthreads();
+  }
   LocalCounters cnt;
   if (kCount)
     for (int i = 0; i < kNumCounters; ++i) cnt.c[i] = 0;
-  const int numLights = countLights(P);
+  const int numLights1 = kBatch ? 0 : countLights(P);
   PathLane L;
   L.st = kStGrab;
+  L.req = 0;
   for (;;) {
     const uint64_t live = __ballot(L.st != kStDone);
     if (!live) break;
     const int nLive = __popcll(live);
     // lanes that can progress without the gated passes
     const bool stuck = __ballot(isTracking(L.st) || L.st == kStAtSample) == 0;
-
     // rare pass, in the order a lane flows through it
     const uint64_t rare = __ballot(isRare(L.st));
     if (rare && (stuck || __popcll(rare) * 64 >= nLive * pp.rareBatch)) {
-      if (L.st == kStSegEnd) segmentEnd<kCount>(L, P, gv, numLights, cnt);
-      if (L.st == kStScatter) scatter(L, P);
-      grabPixels(L, P, pp, lane);
-      if (L.st == kStSampleStart) sampleStart<kCount>(L, P, gv, numLights, cnt);
-      if (L.st == kStPtSegment) beginSegment(L, P, gv, L.ray, kSegPt);
+      {
+        const BatchRequest *R = kBatch ? pp.reqs + L.req : nullptr;
+        const RenderParams &Pr = kBatch ? R->P : P;
+        if (L.st == kStSegEnd) segmentEnd<kCount>(L, Pr, gv, kBatch ? R->numLights : numLights1, cnt);
+        if (L.st == kStScatter) scatter(L, Pr);
+      }
+      grabPixels<kBatch>(L, P, pp, workBegin, lane);
+      // a grab may have moved the lane to another request
+      const BatchRequest *R = kBatch ? pp.reqs + L.req : nullptr;
+      const RenderParams &Pr = kBatch ? R->P : P;
+      if (L.st == kStSampleStart) sampleStart<kCount>(L, Pr, gv, kBatch ? R->numLights : numLights1, cnt);
+      if (L.st == kStPtSegment) beginSegment(L, Pr, gv, L.ray, kSegPt);
+    }
+
+    // this lane's request (after any grab): parameters, majorants and
+    // culling of its transfer function
+    const BatchRequest *R = kBatch ? pp.reqs + L.req : nullptr;
+    const RenderParams &Pl = kBatch ? R->P : P;
+    GridView gvl = gv;
+    Culling cu = cullingOf(sv);
+    if (kBatch) {
+      gvl.majorants = R->majorants;
+      cu.gridletMaxOp = R->gridletMaxOp;
+      cu.elemMaxOp = R->elemMaxOp;
     }
 
     // tracking pass
     for (int n = 0; n < pp.ctrlSteps; ++n) {
       if (!isTracking(L.st)) break;
-      trackingTransition<kCount>(L, P, gv, cnt);
+      trackingTransition<kCount>(L, Pl, gvl, cnt);
     }
 
     // gridlet pass (cached gridlet first)
@@ -291,11 +338,11 @@ kRenderPersistent(RenderParams P, SamplerView sv, GridView gv, PersistParams pp,
     if (L.st == kStAtSample) {
       bool cachedHit;
       if (kAccel == kAccelTables) {
-        smp = lookupTablesGridlets(sv, L.pos, L.primID, cachedHit);
+        smp = lookupTablesGridlets(sv, cu, L.pos, L.primID, cachedHit);
       } else {
         smp = testSampleGridlet(sv, L.pos, L.primID);
         cachedHit = smp.primID >= 0;
-        if (!cachedHit) smp = locateGridlets<kAccel>(sv, L.pos, stack, kPersistBlock);
+        if (!cachedHit) smp = locateGridlets<kAccel>(sv, cu, L.pos, stack, kPersistBlock);
       }
       if (smp.primID >= 0) {
         if (kCount) cnt.c[cachedHit ? kCntCachedHit : kCntGridletHit]++;
@@ -310,7 +357,7 @@ kRenderPersistent(RenderParams P, SamplerView sv, GridView gv, PersistParams pp,
     if (pend && (stuck || __popcll(pend) * 64 >= nLive * pp.elemBatch)) {
       if (L.st == kStPending) {
         int kind;
-        smp = locateElements<kAccel>(sv, L.pos, stack, kPersistBlock, kind);
+        smp = locateElements<kAccel>(sv, cu, L.pos, stack, kPersistBlock, kind);
         if (kCount) cnt.c[kind < 0 ? kCntMiss : kCntGridletHit + kind]++;
         have = true;
       }
@@ -320,7 +367,7 @@ kRenderPersistent(RenderParams P, SamplerView sv, GridView gv, PersistParams pp,
     if (have) {
       if (kCount && smp.primID >= 0) cnt.c[kCntTfFetch]++;
       const bool shadow = L.kind == kSegPtShadow || L.kind == kSegDlShadow;
-      if (resolveCollision(P, smp, shadow, L.majorant, L.rng, L.primID, L.xf, L.Tr)) {
+      if (resolveCollision(Pl, smp, shadow, L.majorant, L.rng, L.primID, L.xf, L.Tr)) {
         L.itype = kScattering;
         L.st = kStSegEnd;
       } else {
@@ -337,13 +384,18 @@ __global__ void kResetWork(uint32_t *work) {
   if (threadIdx.x == 0) *work = 0u;
 }
 
-template <bool kCount, int kAccel>
+__global__ void kAdvanceBatch(BatchRequest *reqs, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) reqs[i].P.accumID += 1;
+}
+
+template <bool kCount, int kAccel, bool kBatch>
 static int residentBlocks() {
   static int cached = 0;
   if (!cached) {
     int perCU = 0, dev = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, kRenderPersistent<kCount, kAccel>, kPersistBlock,
-                                                     0) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, kRenderPersistent<kCount, kAccel, kBatch>,
+                                                     kPersistBlock, 0) != hipSuccess ||
         perCU < 1)
       perCU = 1;
     if (hipGetDevice(&dev) != hipSuccess ||
@@ -354,13 +406,32 @@ static int residentBlocks() {
   return cached;
 }
 
-template <bool kCount, int kAccel>
+template <bool kCount, int kAccel, bool kBatch>
 static void launchPersistentT(const RenderParams &p, const SamplerView &sv, const GridView &gv,
                               const PersistParams &pp, unsigned long long *counters, hipStream_t s) {
-  const int blocks = std::max(1, std::min(residentBlocks<kCount, kAccel>(),
+  const int blocks = std::max(1, std::min(residentBlocks<kCount, kAccel, kBatch>(),
                                           (int)((pp.numWork + kPersistBlock - 1) / kPersistBlock)));
-  hipLaunchKernelGGL((kRenderPersistent<kCount, kAccel>), dim3(blocks), dim3(kPersistBlock), 0, s, p, sv, gv, pp,
-                     counters);
+  hipLaunchKernelGGL((kRenderPersistent<kCount, kAccel, kBatch>), dim3(blocks), dim3(kPersistBlock), 0, s, p, sv,
+                     gv, pp, counters);
+}
+
+template <bool kBatch>
+static void launchAny(const RenderParams &p, const SamplerView &sv, int accel, const GridView &gv,
+                      const PersistParams &pp, unsigned long long *counters, hipStream_t s) {
+  hipLaunchKernelGGL(kResetWork, dim3(1), dim3(64), 0, s, pp.work);
+  if (accel == kAccelBvh) {
+    if (counters) launchPersistentT<true, kAccelBvh, kBatch>(p, sv, gv, pp, counters, s);
+    else launchPersistentT<false, kAccelBvh, kBatch>(p, sv, gv, pp, counters, s);
+  } else {
+    if (counters) launchPersistentT<true, kAccelTables, kBatch>(p, sv, gv, pp, counters, s);
+    else launchPersistentT<false, kAccelTables, kBatch>(p, sv, gv, pp, counters, s);
+  }
+}
+
+static void setTuning(PersistParams &pp, const PersistTuning &t) {
+  pp.ctrlSteps = t.ctrlSteps < 1 ? 1 : t.ctrlSteps;
+  pp.elemBatch = t.elemBatch < 0 ? 0 : (t.elemBatch > 64 ? 64 : t.elemBatch);
+  pp.rareBatch = t.rareBatch < 0 ? 0 : (t.rareBatch > 64 ? 64 : t.rareBatch);
 }
 
 void launchRenderPersistent(const RenderParams &p, const SamplerView &sv, int accel, const GridView &gv,
@@ -368,20 +439,28 @@ void launchRenderPersistent(const RenderParams &p, const SamplerView &sv, int ac
                             hipStream_t s) {
   PersistParams pp;
   pp.work = work;
+  pp.reqs = nullptr;
+  pp.numReqs = 0;
   pp.tilesX = (p.W + kPixelTile - 1) / kPixelTile;
   const uint32_t tilesY = (uint32_t)((p.H + kPixelTile - 1) / kPixelTile);
   pp.numWork = (uint32_t)pp.tilesX * tilesY * 64u;
-  pp.ctrlSteps = t.ctrlSteps < 1 ? 1 : t.ctrlSteps;
-  pp.elemBatch = t.elemBatch < 0 ? 0 : (t.elemBatch > 64 ? 64 : t.elemBatch);
-  pp.rareBatch = t.rareBatch < 0 ? 0 : (t.rareBatch > 64 ? 64 : t.rareBatch);
-  hipLaunchKernelGGL(kResetWork, dim3(1), dim3(64), 0, s, work);
-  if (accel == kAccelBvh) {
-    if (counters) launchPersistentT<true, kAccelBvh>(p, sv, gv, pp, counters, s);
-    else launchPersistentT<false, kAccelBvh>(p, sv, gv, pp, counters, s);
-  } else {
-    if (counters) launchPersistentT<true, kAccelTables>(p, sv, gv, pp, counters, s);
-    else launchPersistentT<false, kAccelTables>(p, sv, gv, pp, counters, s);
-  }
+  setTuning(pp, t);
+  launchAny<false>(p, sv, accel, gv, pp, counters, s);
+}
+
+void launchRenderBatch(const SamplerView &sv, int accel, const GridView &gv, BatchRequest *reqs, int numReqs,
+                       uint32_t numWork, uint32_t *work, const PersistTuning &t, unsigned long long *counters,
+                       hipStream_t s) {
+  PersistParams pp;
+  pp.work = work;
+  pp.reqs = reqs;
+  pp.numReqs = numReqs;
+  pp.tilesX = 0;
+  pp.numWork = numWork;
+  setTuning(pp, t);
+  const RenderParams unused{};  // every lane reads its request's parameters
+  launchAny<true>(unused, sv, accel, gv, pp, counters, s);
+  hipLaunchKernelGGL(kAdvanceBatch, dim3((numReqs + 63) / 64), dim3(64), 0, s, reqs, numReqs);
 }
 
 }  // namespace exa

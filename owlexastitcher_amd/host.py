@@ -416,3 +416,105 @@ def xf_load(path):
     cm = np.empty((n.value, 4), np.float32)
     check(lib().exa_xf_load(path.encode(), None, None, None, None, _ptr(cm), n.value))
     return {"opacityScale": os_.value, "absDomain": ad, "relDomain": rd, "colorMap": cm}
+
+
+class RenderBatch:
+    """Many render requests (views, image sizes, transfer functions) in one
+    persistent launch per frame, sharing a renderer's model and tables.  Each
+    request gives the image the renderer would give for the same settings."""
+
+    COUNTER_NAMES = ("gridlet", "tet", "pyr", "wedge", "hex", "miss", "cached", "dda_cells", "pixels", "tf")
+
+    def __init__(self, renderer):
+        self.renderer = renderer  # keeps the model / sampler alive
+        self.h = C.c_void_p()
+        check(lib().exa_batch_create(renderer.h, C.byref(self.h)))
+        self.sizes = []
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().exa_batch_destroy(self.h)
+            self.h = None
+
+    def addTransferFunction(self, rgba, lo, hi, opacity_scale):
+        """rgba: (N,4) texels; [lo, hi]: absolute domain; opacity_scale multiplies
+        extinction directly (the renderer's slider s stores 1.1**(s-100))."""
+        a = np.ascontiguousarray(rgba, dtype=np.float32)
+        idx = C.c_int()
+        check(lib().exa_batch_add_transfer_function(self.h, _ptr(a), len(a), float(lo), float(hi),
+                                                    float(opacity_scale), C.byref(idx)))
+        return idx.value
+
+    def addRequest(self, width, height, spp, tf, camera, integrator=0):
+        """camera = (org, dir_00, dir_du, dir_dv) as from camera_look_at."""
+        from ._capi import BatchRequestDesc
+        d = BatchRequestDesc()
+        d.width, d.height, d.spp, d.integrator, d.tf = int(width), int(height), int(spp), int(integrator), int(tf)
+        for name, v in zip(("camOrg", "camDir00", "camDu", "camDv"), camera):
+            getattr(d, name)[:] = [float(x) for x in _f32(v, 3)]
+        idx = C.c_int()
+        check(lib().exa_batch_add_request(self.h, C.byref(d), C.byref(idx)))
+        self.sizes.append((int(width), int(height)))
+        return idx.value
+
+    def __len__(self):
+        return len(self.sizes)
+
+    def setTuning(self, ctrl_steps, elem_batch, rare_batch):
+        check(lib().exa_batch_set_tuning(self.h, int(ctrl_steps), int(elem_batch), int(rare_batch)))
+
+    @property
+    def tuning(self):
+        t = np.zeros(3, np.int32)
+        check(lib().exa_batch_get_tuning(self.h, _ptr(t)))
+        return tuple(int(x) for x in t)
+
+    def render(self, stream=None):
+        check(lib().exa_batch_render(self.h, _stream(stream)))
+
+    def render_counted(self, stream=None):
+        c = np.zeros(10, np.uint64)
+        check(lib().exa_batch_render_counted(self.h, _ptr(c), _stream(stream)))
+        return dict(zip(self.COUNTER_NAMES, (int(x) for x in c)))
+
+    def captureGraph(self, stream):
+        check(lib().exa_batch_capture_graph(self.h, _stream(stream)))
+
+    def renderGraph(self, stream=None):
+        check(lib().exa_batch_render_graph(self.h, _stream(stream)))
+
+    def resetAccum(self):
+        check(lib().exa_batch_reset_accum(self.h))
+
+    def accumID(self, i):
+        return check_nonneg(lib().exa_batch_get_accum_id(self.h, int(i)))
+
+    def accum(self, i):
+        w, h = self.sizes[i]
+        out = np.empty((h, w, 4), np.float32)
+        check(lib().exa_batch_get_accum(self.h, int(i), _ptr(out)))
+        return out
+
+    def frame(self, i):
+        w, h = self.sizes[i]
+        out = np.empty((h, w), np.uint32)
+        check(lib().exa_batch_get_frame(self.h, int(i), _ptr(out)))
+        return out
+
+    def autotune(self, ctrl_steps=(4, 8, 16, 32), elem_batch=(0, 8, 16, 32), rare_batch=(0, 8, 16), stream=None):
+        """Times one frame per knob combination, keeps the fastest; resets the
+        accumulation.  Returns (best_knobs, best_ms)."""
+        c = np.ascontiguousarray(ctrl_steps, np.int32)
+        e = np.ascontiguousarray(elem_batch, np.int32)
+        r = np.ascontiguousarray(rare_batch, np.int32)
+        best = np.zeros(3, np.int32)
+        ms = C.c_float()
+        check(lib().exa_batch_autotune(self.h, _ptr(c), len(c), _ptr(e), len(e), _ptr(r), len(r),
+                                       _stream(stream), _ptr(best), C.byref(ms)))
+        return tuple(int(x) for x in best), ms.value
+
+
+def check_nonneg(rc):
+    if rc < 0:
+        check(rc)
+    return rc
