@@ -155,6 +155,99 @@ def read_traffic(args):
     return None
 
 
+def random_tf(rng, n_points):
+    """Random piecewise-linear RGBA transfer function with n_points control
+    points (x ascending from 0 to 1), rasterised to 128 texels."""
+    from owlexastitcher_amd import tf_rasterize
+    xs = np.sort(rng.uniform(0.0, 1.0, n_points - 2))
+    pts = np.zeros((n_points, 5), np.float32)
+    pts[:, 0] = np.concatenate([[0.0], xs, [1.0]])
+    pts[:, 1:4] = rng.uniform(0.0, 1.0, (n_points, 3))
+    pts[:, 4] = rng.uniform(0.0, 1.0, n_points)
+    return tf_rasterize(pts, 128)
+
+
+def batch_workload(r, args, dev):
+    """Batched requests: 64 views, sizes drawn from {256,512,768,1024}^2,
+    seeded orbit cameras, 4 random 5-8 point TFs with opacity scale
+    U[0.5,4], 16 spp each, in one persistent launch per frame."""
+    import torch
+    from owlexastitcher_amd import RENDER_PERSISTENT, RenderBatch, camera_look_at
+    rng = np.random.default_rng(args.seed)
+    vr = r.valueRange
+    tfs = [(random_tf(rng, int(rng.integers(5, 9))), float(rng.uniform(0.5, 4.0))) for _ in range(4)]
+    b = r.modelBounds
+    c = 0.5 * (b[:3] + b[3:])
+    rad = 1.2 * float(np.linalg.norm(b[3:] - b[:3]))
+    reqs = []
+    for _ in range(args.requests):
+        size = int(rng.choice([256, 512, 768, 1024]))
+        az, el = float(rng.uniform(0, 2 * np.pi)), float(rng.uniform(-0.8, 0.8))
+        eye = c + rad * np.array([np.cos(el) * np.cos(az), np.sin(el), np.cos(el) * np.sin(az)])
+        reqs.append((size, int(rng.integers(0, 4)), camera_look_at(eye, c, (0, 1, 0), 45.0, size, size)))
+
+    def make(spp):
+        bt = RenderBatch(r)
+        ti = [bt.addTransferFunction(rgba, vr[0], vr[1], op) for rgba, op in tfs]
+        for size, tf, cam in reqs:
+            bt.addRequest(size, size, spp, ti[tf], cam, args.integrator)
+        return bt
+
+    t = time.time()
+    tune = make(1)
+    best, best_ms = tune.autotune((8, 16, 32), (8, 16, 32), (0, 8, 16))
+    log(f"[bench] autotune on the 1-spp batch: best {best} ({best_ms:.1f} ms/frame), {time.time() - t:.1f}s")
+    del tune
+    bt = make(args.batch_spp)
+    bt.setTuning(*best)
+    stream = torch.cuda.current_stream()
+    for _ in range(args.warmup):
+        bt.render(stream)
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
+    t0 = time.perf_counter()
+    ev[0].record(stream)
+    for i in range(args.steps):
+        bt.render(stream)
+        ev[i + 1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    frame_ms = [ev[i].elapsed_time(ev[i + 1]) for i in range(args.steps)]
+    pixels = sum(q[0] * q[0] for q in reqs)
+    samples = pixels * args.batch_spp
+    # the same requests one at a time through the single-view renderer
+    r.setScheme(RENDER_PERSISTENT)
+    r.setSpp(args.batch_spp)
+    r.setType(args.integrator)
+    single_ms = 0.0
+    fb = None
+    for size, tf, cam in reqs:
+        r.resize(size, size)
+        r.setCamera(*cam)
+        r.setColorMap(tfs[tf][0])
+        r.setRange(vr[0], vr[1])
+        fb = torch.zeros((size, size), dtype=torch.int32, device=dev)
+        r.render(fb, stream)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        r.render(fb, stream)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        single_ms += e0.elapsed_time(e1)
+    log(f"[bench] batch frame {np.mean(frame_ms):.1f} ms vs {single_ms:.1f} ms for the requests one at a time")
+    return {
+        "metric": METRIC, "value": samples * args.steps / wall, "unit": "samples/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": f"{args.requests} batched requests, sizes U{{256,512,768,1024}}^2, seeded orbit "
+                               f"cameras, 4 random 5-8 point TFs (opacity U[0.5,4]), {args.batch_spp} spp, one "
+                               f"persistent launch per frame, same 3-level scene",
+                   "pixels": pixels, "autotuned": list(best), "frame_ms": float(np.mean(frame_ms)),
+                   "single_view_sum_ms": single_ms},
+        "roofline": None, "cpu_baseline": None,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -175,6 +268,10 @@ def main():
     ap.add_argument("--rare-batch", type=int, default=8)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", choices=["frame", "batch"], default="frame",
+                    help="frame: the 1080p frame (default); batch: batched requests in one launch")
+    ap.add_argument("--requests", type=int, default=64)
+    ap.add_argument("--batch-spp", type=int, default=16)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="run one plain device copy of BYTES first (for rocprofv3 counter scaling)")
     args = ap.parse_args()
@@ -192,6 +289,9 @@ def main():
         copy_calibration(args.calibrate_copy, dev)
     amr, dm, model, counts = build_scene(args)
     r, rgba = setup_renderer(model, args)
+    if args.workload == "batch":
+        print(json.dumps(batch_workload(r, args, dev)), flush=True)
+        return
     stream = torch.cuda.current_stream()
     fb = torch.zeros((args.height, args.width), dtype=torch.int32, device=dev)
 
