@@ -155,10 +155,10 @@ int exa_renderer_resize(exa_renderer *r, int width, int height);
    width*height pixels; d_fb may be NULL (accumulation only).  Asynchronous
    on `stream`; accumID advances by one. */
 int exa_renderer_render(exa_renderer *r, uint32_t *d_fb, void *stream);
-/* render() with the instrumented kernel; counters[10] receive: gridlet, tet,
+/* render() with the instrumented kernel; counters[11] receive: gridlet, tet,
    pyramid, wedge, hex hits, misses, cached gridlet hits, DDA cells, pixels,
-   TF fetches (synchronises the stream). */
-int exa_renderer_render_counted(exa_renderer *r, uint32_t *d_fb, uint64_t counters[10], void *stream);
+   TF fetches, neural-field lookups (synchronises the stream). */
+int exa_renderer_render_counted(exa_renderer *r, uint32_t *d_fb, uint64_t counters[11], void *stream);
 int exa_renderer_get_accum(const exa_renderer *r, float *hostOut); /* float4 per pixel */
 int exa_renderer_set_type(exa_renderer *r, int type); /* 0 path tracer, 1 direct lighting */
 int exa_renderer_set_color_map(exa_renderer *r, const float *rgba, int numColors);
@@ -258,7 +258,7 @@ int exa_batch_set_tuning(exa_batch *b, int ctrlSteps, int elemBatch, int rareBat
 int exa_batch_get_tuning(const exa_batch *b, int tuning[3]);
 /* one frame of every request (asynchronous on stream); accumIDs advance */
 int exa_batch_render(exa_batch *b, void *stream);
-int exa_batch_render_counted(exa_batch *b, uint64_t counters[10], void *stream);
+int exa_batch_render_counted(exa_batch *b, uint64_t counters[11], void *stream);
 /* Captures one frame of the batch into a HIP graph on a non-default stream;
    exa_batch_render_graph replays it (the accumIDs live on the device and
    advance inside the graph).  Any change to the batch drops the graph. */
@@ -273,6 +273,46 @@ int exa_batch_get_frame(const exa_batch *b, int req, uint32_t *hostOut);  /* RGB
    accumulation. */
 int exa_batch_autotune(exa_batch *b, const int *ctrlSteps, int nCtrl, const int *elemBatch, int nElem,
                        const int *rareBatch, int nRare, void *stream, int best[3], float *bestMs);
+
+/* ---- neural field sampler (build-side; no reference counterpart) ----
+   A multiresolution hash encoding (levels x 2^log2T entries x 2 bf16
+   features, resolutions nMin..nMax; Mueller et al. 2022) feeding an MLP
+   32-64-64-1 (bf16 weights, float32 master copies, bf16 MFMA with float32
+   accumulation), trained with Adam on samples the exact sampler draws.  After
+   per-macrocell validation it answers the element stage of point location
+   (the stitching elements, ExaStitchSampler.h:128-140) in the macrocells
+   where its max error stays within a bound; the majorants of those
+   macrocells are widened by the bound so that tracking stays conservative. */
+typedef struct exa_neural exa_neural;
+int exa_neural_create(exa_renderer *r, int levels, int log2T, float nMin, float nMax, uint32_t seed,
+                      exa_neural **out);
+void exa_neural_destroy(exa_neural *n);
+/* info[4]: levels, log2T, Adam steps taken, validated; res[16]: level resolutions */
+int exa_neural_get_config(const exa_neural *n, int32_t info[4], int32_t res[16]);
+/* float master parameters: table (levels x 2^log2T x 2), W1 (32x64), W2 (64x64), w3 (64), row-major */
+int exa_neural_get_params(const exa_neural *n, float *table, float *W1, float *W2, float *w3);
+int exa_neural_set_params(exa_neural *n, const float *table, const float *W1, const float *W2, const float *w3);
+/* training pool of count samples drawn with the exact sampler (no culling):
+   elemFraction inside random stitching elements, the rest in random gridlets */
+int exa_neural_make_pool(exa_neural *n, exa_renderer *r, int64_t count, float elemFraction, uint32_t seed,
+                         int64_t *numValid);
+/* Adam steps on random batches of the pool; lossOut[steps] (may be NULL) */
+int exa_neural_train(exa_neural *n, int steps, int batch, float lr, float *lossOut);
+/* loss and gradients of one batch given on the host (x in [0,1]^3 over the
+   model's cell bounds, value normalised to the value range; count % 64 == 0) */
+int exa_neural_gradients(exa_neural *n, const float *x01, const float *target, int count, float *loss,
+                         float *dTable, float *dW1, float *dW2, float *dw3);
+int exa_neural_apply_adam(exa_neural *n, float lr);
+/* values at count device positions (float3, voxel space) */
+int exa_neural_evaluate(const exa_neural *n, const float *d_pos, int64_t count, float *d_out, void *stream);
+/* out[3]: macrocells answered by the network, macrocells with element-stage
+   points, max error over the accepted macrocells (scalar units) */
+int exa_neural_validate(exa_neural *n, exa_renderer *r, int pointsPerMC, float bound, uint32_t seed,
+                        float out[3]);
+int exa_neural_get_flags(const exa_neural *n, uint8_t *out); /* 1 per macrocell */
+/* n validated, or NULL for the exact sampler everywhere (persistent scheme,
+   table accelerator) */
+int exa_renderer_set_neural(exa_renderer *r, exa_neural *n);
 
 /* ---- host helpers ---- */
 /* Camera from eye/point-of-interest/up/fovy for a W x H image

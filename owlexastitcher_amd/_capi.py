@@ -163,6 +163,19 @@ _SIGS = {
     "exa_batch_get_accum": (C.c_int, [P, C.c_int, P]),
     "exa_batch_get_frame": (C.c_int, [P, C.c_int, P]),
     "exa_batch_autotune": (C.c_int, [P, P, C.c_int, P, C.c_int, P, C.c_int, P, P, P]),
+    "exa_neural_create": (C.c_int, [P, C.c_int, C.c_int, F, F, C.c_uint32, P]),
+    "exa_neural_destroy": (None, [P]),
+    "exa_neural_get_config": (C.c_int, [P, P, P]),
+    "exa_neural_get_params": (C.c_int, [P, P, P, P, P]),
+    "exa_neural_set_params": (C.c_int, [P, P, P, P, P]),
+    "exa_neural_make_pool": (C.c_int, [P, P, C.c_int64, F, C.c_uint32, P]),
+    "exa_neural_train": (C.c_int, [P, C.c_int, C.c_int, F, P]),
+    "exa_neural_gradients": (C.c_int, [P, P, P, C.c_int, P, P, P, P, P]),
+    "exa_neural_apply_adam": (C.c_int, [P, F]),
+    "exa_neural_evaluate": (C.c_int, [P, P, C.c_int64, P, P]),
+    "exa_neural_validate": (C.c_int, [P, P, C.c_int, F, C.c_uint32, P]),
+    "exa_neural_get_flags": (C.c_int, [P, P]),
+    "exa_renderer_set_neural": (C.c_int, [P, P]),
     "exa_camera_look_at": (C.c_int, [P, P, P, F, C.c_int, C.c_int, P, P, P, P]),
     "exa_xf_load": (C.c_int, [C.c_char_p, P, P, P, P, P, C.c_int]),
     "exa_tf_rasterize": (C.c_int, [P, C.c_int, C.c_int, P]),

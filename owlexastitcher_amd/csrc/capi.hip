@@ -12,6 +12,7 @@
 #include <vector>
 #include "../../include/exa/exa.h"
 #include "exa_kernels.h"
+#include "neural_kernels.h"
 
 namespace exa {
 
@@ -335,6 +336,59 @@ struct exa_sampler {
   }
 };
 
+// Neural field sampler state: parameters (float master, bf16 copies), Adam
+// moments, the training pool, and the per-macrocell validation result.
+struct exa_neural {
+  int32_t levels = 16, log2T = 19;
+  float nMin = 16.f, nMax = 512.f;
+  int32_t res[kNeuralMaxLevels] = {};
+  uint32_t dense[kNeuralMaxLevels] = {};
+  int64_t entries = 0;  // levels << log2T
+  int64_t numParams = 0;
+  DeviceBuffer<float> master, grad, m, v;  // table (2 per entry) | W1 | W2 | w3
+  DeviceBuffer<uint32_t> table;           // packed bf16 pairs
+  DeviceBuffer<uint16_t> mlp;             // bf16 W1 | W2 | w3
+  DeviceBuffer<float4> pool;
+  int64_t poolSize = 0;
+  DeviceBuffer<float> lossBuf;
+  int step = 0;
+  // model frame
+  float lo[3] = {0, 0, 0}, invExtent[3] = {1, 1, 1};
+  float vmin = 0.f, vrange = 1.f;
+  // validation
+  DeviceBuffer<uint8_t> flags;
+  DeviceBuffer<float2> widened;
+  float bound = 0.f;
+  bool validated = false;
+
+  static constexpr int64_t kW1 = kNeuralIn * kNeuralHidden, kW2 = kNeuralHidden * kNeuralHidden,
+                           kW3 = kNeuralHidden;
+  int64_t mlpOffset() const { return 2 * entries; }
+  NeuralView view() const {
+    NeuralView nv;
+    nv.levels = levels;
+    nv.log2T = log2T;
+    for (int l = 0; l < kNeuralMaxLevels; ++l) {
+      nv.res[l] = res[l];
+      nv.dense[l] = dense[l];
+    }
+    nv.table = table.ptr;
+    nv.W1 = mlp.ptr;
+    nv.W2 = mlp.ptr + kW1;
+    nv.w3 = mlp.ptr + kW1 + kW2;
+    for (int k = 0; k < 3; ++k) {
+      nv.lo[k] = lo[k];
+      nv.invExtent[k] = invExtent[k];
+    }
+    nv.vmin = vmin;
+    nv.vrange = vrange;
+    return nv;
+  }
+  void pack(hipStream_t s) {
+    neuralPack(master.ptr, table.ptr, entries, master.ptr + mlpOffset(), mlp.ptr, kW1 + kW2 + kW3, s);
+  }
+};
+
 struct exa_renderer {
   HostModel model;
   exa_sampler sampler;
@@ -349,6 +403,7 @@ struct exa_renderer {
   int accumID = 0;
   int spp = 1;
   int scheme = EXA_RENDER_PERSISTENT;
+  exa_neural *neural = nullptr;  // neural field for validated macrocells (not owned)
   PersistTuning tuning;
   DeviceBuffer<uint32_t> work;
 
@@ -358,6 +413,15 @@ struct exa_renderer {
     const float lo = absDomain[0] + (relDomain[0] / 100.f) * (absDomain[1] - absDomain[0]);
     const float hi = absDomain[0] + (relDomain[1] / 100.f) * (absDomain[1] - absDomain[0]);
     sampler.computeMaxOpacities(colorMap.data(), (int)colorMap.size() / 4, lo, hi, 0);
+    if (neural && neural->validated) {
+      // majorants of the macrocells the network answers: the TF maximum over
+      // the exact value range widened by the validation bound
+      const int64_t nmc = (int64_t)sampler.dims[0] * sampler.dims[1] * sampler.dims[2];
+      launchMapMaxOpacity(neural->widened.ptr, nmc, sampler.colorMap.ptr, (int)colorMap.size() / 4, lo, hi,
+                          sampler.maxOpacities.ptr, 0);
+      HIP_CHECK(hipGetLastError());
+      HIP_CHECK(hipStreamSynchronize(0));
+    }
   }
 };
 
@@ -783,9 +847,15 @@ static void doRender(exa_renderer *r, uint32_t *fb, unsigned long long *counters
   P.tfN = (int32_t)r->tf.count;
   if (r->scheme == EXA_RENDER_PERSISTENT) {
     r->work.alloc(1);
+    const bool useNet = r->neural && r->neural->validated;
+    if (useNet && r->sampler.accel != kAccelTables)
+      throw Status(EXA_ERR_STATE, "the neural field runs with the table accelerator");
+    NeuralView nv{};
+    if (useNet) nv = r->neural->view();
     launchRenderPersistent(P, r->sampler.view(), r->sampler.accel, r->sampler.gridView(), r->work.ptr, r->tuning,
-                           counters, s);
+                           counters, s, useNet ? &nv : nullptr, useNet ? r->neural->flags.ptr : nullptr);
   } else {
+    if (r->neural && r->neural->validated) throw Status(EXA_ERR_STATE, "the neural field needs the persistent scheme");
     launchRender(P, r->sampler.view(), r->sampler.accel, r->sampler.gridView(), counters, s);
   }
   HIP_CHECK(hipGetLastError());
@@ -797,7 +867,7 @@ int exa_renderer_render(exa_renderer *r, uint32_t *d_fb, void *stream) {
     doRender(r, d_fb, nullptr, (hipStream_t)stream);
   });
 }
-int exa_renderer_render_counted(exa_renderer *r, uint32_t *d_fb, uint64_t counters[10], void *stream) {
+int exa_renderer_render_counted(exa_renderer *r, uint32_t *d_fb, uint64_t counters[11], void *stream) {
   return guarded([&] {
     require(r && counters, "invalid arguments");
     hipStream_t st = (hipStream_t)stream;
@@ -1102,7 +1172,7 @@ int exa_batch_render(exa_batch *b, void *stream) {
     b->advanced();
   });
 }
-int exa_batch_render_counted(exa_batch *b, uint64_t counters[10], void *stream) {
+int exa_batch_render_counted(exa_batch *b, uint64_t counters[11], void *stream) {
   return guarded([&] {
     require(b && counters, "invalid arguments");
     b->sync();
@@ -1219,6 +1289,306 @@ int exa_batch_autotune(exa_batch *b, const int *ctrlSteps, int nCtrl, const int 
     best[1] = bestK.elemBatch;
     best[2] = bestK.rareBatch;
     if (bestMs) *bestMs = bestT;
+  });
+}
+// ---- neural field sampler ---------------------------------------------------------
+int exa_neural_create(exa_renderer *r, int levels, int log2T, float nMin, float nMax, uint32_t seed,
+                      exa_neural **out) {
+  return guarded([&] {
+    require(r && out, "invalid arguments");
+    require(levels >= 1 && levels <= kNeuralMaxLevels, "levels must be in [1, 16]");
+    require(log2T >= 4 && log2T <= 24, "log2T must be in [4, 24]");
+    require(nMin >= 1.f && nMax >= nMin && nMax <= 65536.f, "need 1 <= nMin <= nMax <= 65536");
+    auto n = std::make_unique<exa_neural>();
+    n->levels = levels;
+    n->log2T = log2T;
+    n->nMin = nMin;
+    n->nMax = nMax;
+    const double b = levels > 1 ? exp((log((double)nMax) - log((double)nMin)) / (levels - 1)) : 1.0;
+    for (int l = 0; l < levels; ++l) {
+      n->res[l] = (int32_t)floor((double)nMin * pow(b, (double)l) + 1e-9);
+      const double v = (double)(n->res[l] + 1);
+      n->dense[l] = v * v * v <= (double)(1ll << log2T) ? 1u : 0u;
+    }
+    n->entries = (int64_t)levels << log2T;
+    n->numParams = 2 * n->entries + exa_neural::kW1 + exa_neural::kW2 + exa_neural::kW3;
+    n->master.alloc(n->numParams);
+    n->grad.alloc(n->numParams);
+    n->m.alloc(n->numParams);
+    n->v.alloc(n->numParams);
+    n->table.alloc(n->entries);
+    n->mlp.alloc(exa_neural::kW1 + exa_neural::kW2 + exa_neural::kW3);
+    float *p = n->master.ptr;
+    neuralUniform(p, 2 * n->entries, -1e-4f, 1e-4f, seed, 0);
+    p += 2 * n->entries;
+    const float a1 = sqrtf(6.f / (kNeuralIn + kNeuralHidden)), a2 = sqrtf(6.f / (2 * kNeuralHidden)),
+                a3 = sqrtf(6.f / (kNeuralHidden + 1));
+    neuralUniform(p, exa_neural::kW1, -a1, a1, seed + 1, 0);
+    neuralUniform(p + exa_neural::kW1, exa_neural::kW2, -a2, a2, seed + 2, 0);
+    neuralUniform(p + exa_neural::kW1 + exa_neural::kW2, exa_neural::kW3, -a3, a3, seed + 3, 0);
+    neuralZero(n->m.ptr, n->numParams, 0);
+    neuralZero(n->v.ptr, n->numParams, 0);
+    n->pack(0);
+    // model frame: positions normalised over the cell bounds, values over the value range
+    const HostModel &M = r->model;
+    for (int k = 0; k < 3; ++k) {
+      n->lo[k] = M.cellBounds[k];
+      const float ext = M.cellBounds[3 + k] - M.cellBounds[k];
+      n->invExtent[k] = ext > 0.f ? 1.f / ext : 1.f;
+    }
+    n->vmin = M.valueRange[0];
+    n->vrange = M.valueRange[1] > M.valueRange[0] ? M.valueRange[1] - M.valueRange[0] : 1.f;
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipDeviceSynchronize());
+    n->validated = false;
+    *out = n.release();
+  });
+}
+void exa_neural_destroy(exa_neural *n) { delete n; }
+int exa_neural_get_config(const exa_neural *n, int32_t info[4], int32_t res[16]) {
+  return guarded([&] {
+    require(n && info, "invalid arguments");
+    info[0] = n->levels;
+    info[1] = n->log2T;
+    info[2] = n->step;
+    info[3] = n->validated ? 1 : 0;
+    if (res)
+      for (int l = 0; l < kNeuralMaxLevels; ++l) res[l] = n->res[l];
+  });
+}
+// parameters as float: table (levels x 2^log2T x 2), W1 (32 x 64), W2 (64 x 64), w3 (64)
+int exa_neural_get_params(const exa_neural *n, float *table, float *W1, float *W2, float *w3) {
+  return guarded([&] {
+    require(n, "neural field is NULL");
+    HIP_CHECK(hipDeviceSynchronize());
+    const float *p = n->master.ptr;
+    if (table) HIP_CHECK(hipMemcpy(table, p, 2 * n->entries * sizeof(float), hipMemcpyDeviceToHost));
+    p += 2 * n->entries;
+    if (W1) HIP_CHECK(hipMemcpy(W1, p, exa_neural::kW1 * sizeof(float), hipMemcpyDeviceToHost));
+    if (W2) HIP_CHECK(hipMemcpy(W2, p + exa_neural::kW1, exa_neural::kW2 * sizeof(float), hipMemcpyDeviceToHost));
+    if (w3)
+      HIP_CHECK(hipMemcpy(w3, p + exa_neural::kW1 + exa_neural::kW2, exa_neural::kW3 * sizeof(float),
+                          hipMemcpyDeviceToHost));
+  });
+}
+int exa_neural_set_params(exa_neural *n, const float *table, const float *W1, const float *W2, const float *w3) {
+  return guarded([&] {
+    require(n && table && W1 && W2 && w3, "invalid arguments");
+    float *p = n->master.ptr;
+    HIP_CHECK(hipMemcpy(p, table, 2 * n->entries * sizeof(float), hipMemcpyHostToDevice));
+    p += 2 * n->entries;
+    HIP_CHECK(hipMemcpy(p, W1, exa_neural::kW1 * sizeof(float), hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(p + exa_neural::kW1, W2, exa_neural::kW2 * sizeof(float), hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(p + exa_neural::kW1 + exa_neural::kW2, w3, exa_neural::kW3 * sizeof(float),
+                        hipMemcpyHostToDevice));
+    n->pack(0);
+    neuralZero(n->m.ptr, n->numParams, 0);
+    neuralZero(n->v.ptr, n->numParams, 0);
+    n->step = 0;
+    n->validated = false;
+    HIP_CHECK(hipDeviceSynchronize());
+  });
+}
+// Training pool of n samples drawn with the exact sampler (culling off):
+// elemFrac of the points inside random stitching elements, the rest inside
+// random gridlets.  Returns the number of points the exact sampler found.
+int exa_neural_make_pool(exa_neural *n, exa_renderer *r, int64_t count, float elemFrac, uint32_t seed,
+                         int64_t *numValid) {
+  return guarded([&] {
+    require(n && r && count > 0 && count < (1ll << 31), "invalid arguments");
+    exa_sampler &S = r->sampler;
+    DeviceBuffer<float> pos;
+    DeviceBuffer<Sample> smp;
+    pos.alloc(3 * count);
+    smp.alloc(count);
+    neuralPoolPositions(S.indices.ptr, S.vertices.ptr, S.numElements, S.gridlets.ptr, S.numGridlets, count,
+                        elemFrac, seed, pos.ptr, 0);
+    SamplerView sv = S.view();
+    sv.gridletMaxOp = nullptr;
+    sv.elemMaxOp = nullptr;
+    launchLocate(sv, S.accel, pos.ptr, count, smp.ptr, nullptr, 0);
+    n->pool.alloc(count);
+    neuralPoolFinish(pos.ptr, smp.ptr, count, n->view(), n->pool.ptr, 0);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipDeviceSynchronize());
+    n->poolSize = count;
+    if (numValid) {
+      std::vector<float4> h(count);
+      HIP_CHECK(hipMemcpy(h.data(), n->pool.ptr, count * sizeof(float4), hipMemcpyDeviceToHost));
+      int64_t c = 0;
+      for (const float4 &e : h) c += e.w == e.w ? 1 : 0;
+      *numValid = c;
+    }
+  });
+}
+static void neuralStep(exa_neural *n, const float4 *pool, int64_t poolSize, int batch, bool identity,
+                       uint32_t seed, float *lossSlot, hipStream_t s) {
+  neuralZero(n->grad.ptr, n->numParams, s);
+  TrainArgs a;
+  a.nv = n->view();
+  a.pool = pool;
+  a.poolSize = poolSize;
+  a.batch = batch;
+  a.tilesPerWave = identity ? 1 : 8;
+  a.identity = identity ? 1 : 0;
+  a.stepSeed = seed;
+  a.gTable = n->grad.ptr;
+  a.gW1 = n->grad.ptr + n->mlpOffset();
+  a.gW2 = a.gW1 + exa_neural::kW1;
+  a.gw3 = a.gW2 + exa_neural::kW2;
+  a.lossSum = lossSlot;
+  neuralTrain(a, s);
+}
+// steps of Adam on batches drawn from the pool; lossOut (may be NULL) receives
+// the mean squared error of each step's batch (normalised values)
+int exa_neural_train(exa_neural *n, int steps, int batch, float lr, float *lossOut) {
+  return guarded([&] {
+    require(n && steps >= 0 && batch >= 64 && batch % 64 == 0, "batch must be a positive multiple of 64");
+    require(n->poolSize > 0, "no training pool (exa_neural_make_pool)");
+    n->lossBuf.alloc(std::max(steps, 1));
+    neuralZero(n->lossBuf.ptr, std::max(steps, 1), 0);
+    for (int i = 0; i < steps; ++i) {
+      n->step++;
+      neuralStep(n, n->pool.ptr, n->poolSize, batch, false, 0x85ebca6bu * (uint32_t)n->step, n->lossBuf.ptr + i, 0);
+      neuralAdam(n->master.ptr, n->grad.ptr, n->m.ptr, n->v.ptr, n->numParams, lr, 0.9f, 0.99f, 1e-10f, n->step,
+                 0);
+      n->pack(0);
+    }
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipDeviceSynchronize());
+    n->validated = false;
+    if (lossOut && steps > 0) {
+      HIP_CHECK(hipMemcpy(lossOut, n->lossBuf.ptr, steps * sizeof(float), hipMemcpyDeviceToHost));
+      for (int i = 0; i < steps; ++i) lossOut[i] /= (float)batch;
+    }
+  });
+}
+// Loss and gradients of one batch given on the host (x in [0,1]^3, targets
+// normalised; NaN target = no sample), rows used in order.
+int exa_neural_gradients(exa_neural *n, const float *x01, const float *target, int count, float *loss,
+                         float *dTable, float *dW1, float *dW2, float *dw3) {
+  return guarded([&] {
+    require(n && x01 && target && count > 0 && count % 64 == 0, "count must be a positive multiple of 64");
+    std::vector<float4> h(count);
+    for (int i = 0; i < count; ++i) h[i] = make_float4(x01[3 * i], x01[3 * i + 1], x01[3 * i + 2], target[i]);
+    DeviceBuffer<float4> pool;
+    pool.upload(h.data(), count, 0);
+    DeviceBuffer<float> ls;
+    ls.alloc(1);
+    neuralZero(ls.ptr, 1, 0);
+    neuralStep(n, pool.ptr, count, count, true, 0, ls.ptr, 0);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipDeviceSynchronize());
+    float l = 0.f;
+    HIP_CHECK(hipMemcpy(&l, ls.ptr, sizeof(float), hipMemcpyDeviceToHost));
+    if (loss) *loss = l / (float)count;
+    const float *g = n->grad.ptr;
+    if (dTable) HIP_CHECK(hipMemcpy(dTable, g, 2 * n->entries * sizeof(float), hipMemcpyDeviceToHost));
+    g += 2 * n->entries;
+    if (dW1) HIP_CHECK(hipMemcpy(dW1, g, exa_neural::kW1 * sizeof(float), hipMemcpyDeviceToHost));
+    if (dW2) HIP_CHECK(hipMemcpy(dW2, g + exa_neural::kW1, exa_neural::kW2 * sizeof(float), hipMemcpyDeviceToHost));
+    if (dw3)
+      HIP_CHECK(hipMemcpy(dw3, g + exa_neural::kW1 + exa_neural::kW2, exa_neural::kW3 * sizeof(float),
+                          hipMemcpyDeviceToHost));
+  });
+}
+// one Adam step with the gradients of the last exa_neural_gradients call
+int exa_neural_apply_adam(exa_neural *n, float lr) {
+  return guarded([&] {
+    require(n, "neural field is NULL");
+    n->step++;
+    neuralAdam(n->master.ptr, n->grad.ptr, n->m.ptr, n->v.ptr, n->numParams, lr, 0.9f, 0.99f, 1e-10f, n->step, 0);
+    n->pack(0);
+    HIP_CHECK(hipDeviceSynchronize());
+  });
+}
+int exa_neural_evaluate(const exa_neural *n, const float *d_pos, int64_t count, float *d_out, void *stream) {
+  return guarded([&] {
+    require(n && (count == 0 || (d_pos && d_out)) && count >= 0, "invalid arguments");
+    neuralEval(n->view(), d_pos, count, d_out, (hipStream_t)stream);
+    HIP_CHECK(hipGetLastError());
+  });
+}
+// Per-macrocell validation: pointsPerMC uniform points in every macrocell; a
+// macrocell is answered by the network when it had points that reach the
+// element stage, none that the exact sampler misses, and a max |error| <=
+// bound (scalar units).  out[3]: macrocells using the network, macrocells with
+// element-stage points, max error over the accepted macrocells.
+int exa_neural_validate(exa_neural *n, exa_renderer *r, int pointsPerMC, float bound, uint32_t seed,
+                        float out[3]) {
+  return guarded([&] {
+    require(n && r && pointsPerMC > 0 && bound >= 0.f, "invalid arguments");
+    exa_sampler &S = r->sampler;
+    const GridView gv = S.gridView();
+    const int64_t nmc = (int64_t)gv.dims[0] * gv.dims[1] * gv.dims[2];
+    const int64_t np = nmc * pointsPerMC;
+    require(np < (1ll << 31), "too many validation points");
+    DeviceBuffer<float> pos, net;
+    DeviceBuffer<Sample> smp;
+    DeviceBuffer<uint32_t> cnt, miss, err;
+    pos.alloc(3 * np);
+    net.alloc(np);
+    smp.alloc(np);
+    cnt.alloc(nmc);
+    miss.alloc(nmc);
+    err.alloc(nmc);
+    HIP_CHECK(hipMemset(cnt.ptr, 0, nmc * 4));
+    HIP_CHECK(hipMemset(miss.ptr, 0, nmc * 4));
+    HIP_CHECK(hipMemset(err.ptr, 0, nmc * 4));
+    neuralValidate(gv, pointsPerMC, seed, pos.ptr, np, 0);
+    SamplerView sv = S.view();
+    sv.gridletMaxOp = nullptr;
+    sv.elemMaxOp = nullptr;
+    launchLocate(sv, S.accel, pos.ptr, np, smp.ptr, nullptr, 0);
+    neuralEval(n->view(), pos.ptr, np, net.ptr, 0);
+    neuralValReduce(smp.ptr, net.ptr, np, pointsPerMC, cnt.ptr, miss.ptr, err.ptr, 0);
+    n->flags.alloc(nmc);
+    neuralValFlags(cnt.ptr, miss.ptr, err.ptr, nmc, bound, n->flags.ptr, 0);
+    n->widened.alloc(nmc);
+    neuralWidenRanges(S.valueRanges.ptr, n->flags.ptr, nmc, bound, n->widened.ptr, 0);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipDeviceSynchronize());
+    std::vector<uint32_t> hc(nmc), he(nmc);
+    std::vector<uint8_t> hf(nmc);
+    HIP_CHECK(hipMemcpy(hc.data(), cnt.ptr, nmc * 4, hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpy(he.data(), err.ptr, nmc * 4, hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpy(hf.data(), n->flags.ptr, nmc, hipMemcpyDeviceToHost));
+    int64_t used = 0, withElems = 0;
+    float maxErr = 0.f;
+    for (int64_t i = 0; i < nmc; ++i) {
+      withElems += hc[i] > 0 ? 1 : 0;
+      if (hf[i]) {
+        used++;
+        float e;
+        memcpy(&e, &he[i], 4);
+        maxErr = std::max(maxErr, e);
+      }
+    }
+    n->bound = bound;
+    n->validated = true;
+    if (out) {
+      out[0] = (float)used;
+      out[1] = (float)withElems;
+      out[2] = maxErr;
+    }
+    if (r->neural == n) r->updateMaxOpacities();
+  });
+}
+int exa_neural_get_flags(const exa_neural *n, uint8_t *out) {
+  return guarded([&] {
+    require(n && out && n->validated, "not validated");
+    download(n->flags, out);
+  });
+}
+// element-stage lookups of validated macrocells answered by the network
+// (NULL: exact sampler everywhere); majorants are rebuilt accordingly
+int exa_renderer_set_neural(exa_renderer *r, exa_neural *n) {
+  return guarded([&] {
+    require(r, "renderer is NULL");
+    require(!n || n->validated, "the neural field must be validated first (exa_neural_validate)");
+    r->neural = n;
+    r->updateMaxOpacities();
+    r->accumID = 0;
   });
 }
 int exa_renderer_get_state(const exa_renderer *r, exa_render_state *o) {

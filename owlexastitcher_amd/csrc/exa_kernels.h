@@ -65,6 +65,7 @@ enum Counter {
   kCntDdaCells,
   kCntPixels,
   kCntTfFetch,
+  kCntNeural,  // element-stage lookups answered by the neural field
   kNumCounters
 };
 
@@ -145,9 +146,12 @@ struct PersistTuning {
   int32_t rareBatch = 8;   // per-segment pass when waiting lanes >= rareBatch/64 of live lanes
 };
 // work: one device uint32 (reset by the launch itself)
+struct NeuralView;
+// nv / netFlags (optional, tables accelerator only): macrocells flagged 1
+// answer element-stage lookups with the neural field
 void launchRenderPersistent(const RenderParams &p, const SamplerView &sv, int accel, const GridView &gv,
-                            uint32_t *work, const PersistTuning &t, unsigned long long *counters,
-                            hipStream_t s);
+                            uint32_t *work, const PersistTuning &t, unsigned long long *counters, hipStream_t s,
+                            const NeuralView *nv = nullptr, const uint8_t *netFlags = nullptr);
 // Batched launch over reqs[0..numReqs) (device array); numWork = total work
 // ids.  Afterwards every request's accumID is advanced by one on the device.
 void launchRenderBatch(const SamplerView &sv, int accel, const GridView &gv, BatchRequest *reqs, int numReqs,

@@ -28,6 +28,7 @@
 // so every pixel's result is bit-identical to it; only the interleaving of
 // lanes changes.
 #include "exa_render_common.h"
+#include "neural.h"
 
 namespace exa {
 
@@ -81,6 +82,8 @@ struct PathLane {
 };
 
 struct PersistParams {
+  NeuralView nv;              // neural field (kNeural launches)
+  const uint8_t *netFlags;    // per macrocell: 1 = element stage answered by the network
   uint32_t *work;     // frame-wide pixel-tile counter, zero at launch
   uint32_t numWork;   // 64 per 8x8 pixel tile (of all requests)
   const BatchRequest *reqs;  // batched launch: the requests
@@ -273,17 +276,27 @@ EXA_DEV void grabPixels(PathLane &L, const RenderParams &P, const PersistParams 
   L.st = kStSampleStart;
 }
 
-template <bool kCount, int kAccel, bool kBatch>
+template <bool kCount, int kAccel, bool kBatch, bool kNeural>
 __global__ __launch_bounds__(kPersistBlock) __attribute__((amdgpu_waves_per_eu(EXA_PERSIST_MIN_WAVES, 8))) void
 kRenderPersistent(RenderParams P, SamplerView sv, GridView gv, PersistParams pp, unsigned long long *counters) {
   __shared__ int32_t stackMem[kAccel == kAccelBvh ? kStackDepth * kPersistBlock : 1];
   __shared__ uint32_t workBegin[kBatch ? kMaxBatchRequests : 1];
+  // neural field: bf16 weights, and per wave the encodings and first hidden layer
+  __shared__ uint16_t nW1[kNeural ? kNeuralIn * kNeuralHidden : 1], nW2[kNeural ? kNeuralHidden * kNeuralHidden : 1];
+  __shared__ uint16_t nX[kNeural ? (kPersistBlock / 64) * 64 * kNeuralIn : 1];
+  __shared__ uint16_t nH1[kNeural ? (kPersistBlock / 64) * 64 * kNeuralHidden : 1];
   int32_t *stack = stackMem + threadIdx.x;
   const int lane = (int)__lane_id();
   if (kBatch) {
     for (int i = threadIdx.x; i < pp.numReqs; i += kPersistBlock) workBegin[i] = pp.reqs[i].workBegin;
     __syncthreads();
   }
+  if (kNeural) {
+    for (int i = threadIdx.x; i < kNeuralIn * kNeuralHidden; i += kPersistBlock) nW1[i] = pp.nv.W1[i];
+    for (int i = threadIdx.x; i < kNeuralHidden * kNeuralHidden; i += kPersistBlock) nW2[i] = pp.nv.W2[i];
+    __syncthreads();
+  }
+  const int waveInBlock = (int)threadIdx.x >> 6;
   LocalCounters cnt;
   if (kCount)
     for (int i = 0; i < kNumCounters; ++i) cnt.c[i] = 0;
@@ -355,7 +368,21 @@ kRenderPersistent(RenderParams P, SamplerView sv, GridView gv, PersistParams pp,
     // element pass
     const uint64_t pend = __ballot(L.st == kStPending);
     if (pend && (stuck || __popcll(pend) * 64 >= nLive * pp.elemBatch)) {
-      if (L.st == kStPending) {
+      bool net = false;
+      if (kNeural) {
+        // validated macrocells: the neural field answers (whole wave, MFMA)
+        net = L.st == kStPending && pp.netFlags[segmentCellID(gvl, L.S)];
+        if (__ballot(net)) {
+          const float v = neuralValue(pp.nv, L.pos, nW1, nW2, nX + waveInBlock * 64 * kNeuralIn,
+                                      nH1 + waveInBlock * 64 * kNeuralHidden);
+          if (net) {
+            smp = Sample{0x7fffffff, -1, v};
+            if (kCount) cnt.c[kCntNeural]++;
+            have = true;
+          }
+        }
+      }
+      if (L.st == kStPending && !net) {
         int kind;
         smp = locateElements<kAccel>(sv, cu, L.pos, stack, kPersistBlock, kind);
         if (kCount) cnt.c[kind < 0 ? kCntMiss : kCntGridletHit + kind]++;
@@ -389,12 +416,12 @@ __global__ void kAdvanceBatch(BatchRequest *reqs, int n) {
   if (i < n) reqs[i].P.accumID += 1;
 }
 
-template <bool kCount, int kAccel, bool kBatch>
+template <bool kCount, int kAccel, bool kBatch, bool kNeural>
 static int residentBlocks() {
   static int cached = 0;
   if (!cached) {
     int perCU = 0, dev = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, kRenderPersistent<kCount, kAccel, kBatch>,
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, kRenderPersistent<kCount, kAccel, kBatch, kNeural>,
                                                      kPersistBlock, 0) != hipSuccess ||
         perCU < 1)
       perCU = 1;
@@ -406,19 +433,24 @@ static int residentBlocks() {
   return cached;
 }
 
-template <bool kCount, int kAccel, bool kBatch>
+template <bool kCount, int kAccel, bool kBatch, bool kNeural = false>
 static void launchPersistentT(const RenderParams &p, const SamplerView &sv, const GridView &gv,
                               const PersistParams &pp, unsigned long long *counters, hipStream_t s) {
-  const int blocks = std::max(1, std::min(residentBlocks<kCount, kAccel, kBatch>(),
+  const int blocks = std::max(1, std::min(residentBlocks<kCount, kAccel, kBatch, kNeural>(),
                                           (int)((pp.numWork + kPersistBlock - 1) / kPersistBlock)));
-  hipLaunchKernelGGL((kRenderPersistent<kCount, kAccel, kBatch>), dim3(blocks), dim3(kPersistBlock), 0, s, p, sv,
-                     gv, pp, counters);
+  hipLaunchKernelGGL((kRenderPersistent<kCount, kAccel, kBatch, kNeural>), dim3(blocks), dim3(kPersistBlock), 0, s,
+                     p, sv, gv, pp, counters);
 }
 
 template <bool kBatch>
 static void launchAny(const RenderParams &p, const SamplerView &sv, int accel, const GridView &gv,
                       const PersistParams &pp, unsigned long long *counters, hipStream_t s) {
   hipLaunchKernelGGL(kResetWork, dim3(1), dim3(64), 0, s, pp.work);
+  if (!kBatch && pp.netFlags) {  // neural field: tables accelerator, single view
+    if (counters) launchPersistentT<true, kAccelTables, false, true>(p, sv, gv, pp, counters, s);
+    else launchPersistentT<false, kAccelTables, false, true>(p, sv, gv, pp, counters, s);
+    return;
+  }
   if (accel == kAccelBvh) {
     if (counters) launchPersistentT<true, kAccelBvh, kBatch>(p, sv, gv, pp, counters, s);
     else launchPersistentT<false, kAccelBvh, kBatch>(p, sv, gv, pp, counters, s);
@@ -435,9 +467,12 @@ static void setTuning(PersistParams &pp, const PersistTuning &t) {
 }
 
 void launchRenderPersistent(const RenderParams &p, const SamplerView &sv, int accel, const GridView &gv,
-                            uint32_t *work, const PersistTuning &t, unsigned long long *counters,
-                            hipStream_t s) {
+                            uint32_t *work, const PersistTuning &t, unsigned long long *counters, hipStream_t s,
+                            const NeuralView *nv, const uint8_t *netFlags) {
   PersistParams pp;
+  pp.nv = NeuralView{};
+  if (nv) pp.nv = *nv;
+  pp.netFlags = nv ? netFlags : nullptr;
   pp.work = work;
   pp.reqs = nullptr;
   pp.numReqs = 0;
@@ -452,6 +487,8 @@ void launchRenderBatch(const SamplerView &sv, int accel, const GridView &gv, Bat
                        uint32_t numWork, uint32_t *work, const PersistTuning &t, unsigned long long *counters,
                        hipStream_t s) {
   PersistParams pp;
+  pp.nv = NeuralView{};
+  pp.netFlags = nullptr;
   pp.work = work;
   pp.reqs = reqs;
   pp.numReqs = numReqs;

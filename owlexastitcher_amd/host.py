@@ -282,9 +282,9 @@ class OWLRenderer:
 
     def render_counted(self, d_fb=None, stream=None):
         ptr = C.c_void_p(d_fb.data_ptr()) if d_fb is not None else None
-        c = np.zeros(10, np.uint64)
+        c = np.zeros(11, np.uint64)
         check(lib().exa_renderer_render_counted(self.h, ptr, _ptr(c), _stream(stream)))
-        names = ("gridlet", "tet", "pyr", "wedge", "hex", "miss", "cached", "dda_cells", "pixels", "tf")
+        names = ("gridlet", "tet", "pyr", "wedge", "hex", "miss", "cached", "dda_cells", "pixels", "tf", "neural")
         return dict(zip(names, (int(x) for x in c)))
 
     def accum(self):
@@ -343,6 +343,12 @@ class OWLRenderer:
 
     def setAccel(self, accel):
         check(lib().exa_renderer_set_accel(self.h, int(accel)))
+
+    def setNeural(self, field):
+        """Use a validated NeuralField for the element stage of its macrocells
+        (None: exact sampler everywhere)."""
+        check(lib().exa_renderer_set_neural(self.h, field.h if field is not None else None))
+        self.neural = field  # keep it alive while in use
 
     def setScheme(self, scheme):
         """RENDER_PERSISTENT (default) or RENDER_MEGAKERNEL; same images."""
@@ -418,12 +424,99 @@ def xf_load(path):
     return {"opacityScale": os_.value, "absDomain": ad, "relDomain": rd, "colorMap": cm}
 
 
+class NeuralField:
+    """Neural field sampler: multiresolution hash encoding (levels x 2^log2T
+    entries x 2 bf16 features) + MLP 32-64-64-1, trained on samples of the
+    renderer's exact sampler; per-macrocell validation decides where it stands
+    in for element point location."""
+
+    def __init__(self, renderer, levels=16, log2T=19, n_min=16.0, n_max=512.0, seed=1):
+        self.renderer = renderer
+        self.h = C.c_void_p()
+        check(lib().exa_neural_create(renderer.h, int(levels), int(log2T), float(n_min), float(n_max),
+                                      int(seed), C.byref(self.h)))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().exa_neural_destroy(self.h)
+            self.h = None
+
+    def config(self):
+        info = np.zeros(4, np.int32)
+        res = np.zeros(16, np.int32)
+        check(lib().exa_neural_get_config(self.h, _ptr(info), _ptr(res)))
+        return {"levels": int(info[0]), "log2T": int(info[1]), "step": int(info[2]), "validated": bool(info[3]),
+                "res": [int(x) for x in res[:info[0]]]}
+
+    def params(self):
+        c = self.config()
+        table = np.empty((c["levels"], 1 << c["log2T"], 2), np.float32)
+        W1 = np.empty((32, 64), np.float32)
+        W2 = np.empty((64, 64), np.float32)
+        w3 = np.empty(64, np.float32)
+        check(lib().exa_neural_get_params(self.h, _ptr(table), _ptr(W1), _ptr(W2), _ptr(w3)))
+        return table, W1, W2, w3
+
+    def setParams(self, table, W1, W2, w3):
+        arrs = [np.ascontiguousarray(a, np.float32) for a in (table, W1, W2, w3)]
+        check(lib().exa_neural_set_params(self.h, *[_ptr(a) for a in arrs]))
+
+    def makePool(self, count, elem_fraction=0.7, seed=7):
+        nv = C.c_int64()
+        check(lib().exa_neural_make_pool(self.h, self.renderer.h, int(count), float(elem_fraction), int(seed),
+                                         C.byref(nv)))
+        return nv.value
+
+    def train(self, steps, batch=1 << 18, lr=1e-2):
+        loss = np.zeros(max(int(steps), 1), np.float32)
+        check(lib().exa_neural_train(self.h, int(steps), int(batch), float(lr), _ptr(loss)))
+        return loss[:int(steps)]
+
+    def gradients(self, x01, target):
+        """Loss and gradients of one batch (rows in order; len % 64 == 0)."""
+        c = self.config()
+        x = np.ascontiguousarray(x01, np.float32)
+        t = np.ascontiguousarray(target, np.float32)
+        loss = C.c_float()
+        dt = np.empty((c["levels"], 1 << c["log2T"], 2), np.float32)
+        d1 = np.empty((32, 64), np.float32)
+        d2 = np.empty((64, 64), np.float32)
+        d3 = np.empty(64, np.float32)
+        check(lib().exa_neural_gradients(self.h, _ptr(x), _ptr(t), len(t), C.byref(loss), _ptr(dt), _ptr(d1),
+                                         _ptr(d2), _ptr(d3)))
+        return loss.value, (dt, d1, d2, d3)
+
+    def applyAdam(self, lr):
+        check(lib().exa_neural_apply_adam(self.h, float(lr)))
+
+    def evaluate(self, d_pos, d_out, stream=None):
+        """Values at device positions (n x 3 float32, voxel space)."""
+        check(lib().exa_neural_evaluate(self.h, C.c_void_p(d_pos.data_ptr()), d_pos.shape[0],
+                                        C.c_void_p(d_out.data_ptr()), _stream(stream)))
+
+    def validate(self, points_per_mc=64, bound=None, seed=11):
+        """bound: max abs error in scalar units (default 1e-3 of the value range)."""
+        if bound is None:
+            vr = self.renderer.valueRange
+            bound = 1e-3 * float(vr[1] - vr[0])
+        out = np.zeros(3, np.float32)
+        check(lib().exa_neural_validate(self.h, self.renderer.h, int(points_per_mc), float(bound), int(seed),
+                                        _ptr(out)))
+        return {"neural_mcs": int(out[0]), "element_mcs": int(out[1]), "max_error": float(out[2]), "bound": bound}
+
+    def flags(self, num_mcs):
+        f = np.zeros(num_mcs, np.uint8)
+        check(lib().exa_neural_get_flags(self.h, _ptr(f)))
+        return f
+
+
 class RenderBatch:
     """Many render requests (views, image sizes, transfer functions) in one
     persistent launch per frame, sharing a renderer's model and tables.  Each
     request gives the image the renderer would give for the same settings."""
 
-    COUNTER_NAMES = ("gridlet", "tet", "pyr", "wedge", "hex", "miss", "cached", "dda_cells", "pixels", "tf")
+    COUNTER_NAMES = ("gridlet", "tet", "pyr", "wedge", "hex", "miss", "cached", "dda_cells", "pixels", "tf",
+                     "neural")
 
     def __init__(self, renderer):
         self.renderer = renderer  # keeps the model / sampler alive
@@ -473,7 +566,7 @@ class RenderBatch:
         check(lib().exa_batch_render(self.h, _stream(stream)))
 
     def render_counted(self, stream=None):
-        c = np.zeros(10, np.uint64)
+        c = np.zeros(11, np.uint64)
         check(lib().exa_batch_render_counted(self.h, _ptr(c), _stream(stream)))
         return dict(zip(self.COUNTER_NAMES, (int(x) for x in c)))
 
