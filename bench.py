@@ -28,7 +28,7 @@ HBM_PEAK_GBS = 8000.0
 # index row + 16 B per vertex; a cached gridlet hit re-reads the same 64 B;
 # 4 B per macrocell visited; 36 B per pixel (accum read + write + fb write)
 BYTES = {"gridlet": 64, "tet": 96, "pyr": 112, "wedge": 128, "hex": 160, "cached": 64, "dda_cells": 4,
-         "pixels": 36}
+         "pixels": 36, "neural": 512}
 
 TF5 = np.array([
     [0.00, 0.0, 0.0, 0.3, 0.00],
@@ -248,6 +248,85 @@ def batch_workload(r, args, dev):
     }
 
 
+def neural_workload(r, args, dev):
+    """Learned sampler: hash grid (16 levels x 2^19 x 2) + MLP 32-64-64-1,
+    trained with Adam on samples of the exact sampler (70% in stitching
+    elements, 30% in gridlets), validated per macrocell against a bound of
+    1e-3 of the value range; then the same frames rendered with the exact
+    sampler and with the network answering the element stage."""
+    import torch
+    from owlexastitcher_amd import NeuralField
+    vr = r.valueRange
+    f = NeuralField(r, levels=16, log2T=19, n_min=16, n_max=512, seed=args.seed)
+    t = time.time()
+    nvalid = f.makePool(args.pool, 0.7, seed=args.seed + 6)
+    log(f"[bench] pool: {nvalid} of {args.pool} samples found by the exact sampler ({time.time() - t:.1f}s)")
+    torch.cuda.synchronize()
+    t = time.time()
+    loss = f.train(args.train_steps, batch=args.train_batch, lr=1e-2, lr_end=args.lr_end)
+    train_s = time.time() - t
+    log(f"[bench] trained {args.train_steps} steps x {args.train_batch} in {train_s:.1f}s; "
+        f"loss {loss[0]:.3g} -> {np.mean(loss[-50:]):.3g}")
+    t = time.time()
+    v = f.validate(points_per_mc=args.val_points, bound=args.val_bound * float(vr[1] - vr[0]))
+    log(f"[bench] validation ({time.time() - t:.1f}s): {v}")
+    stream = torch.cuda.current_stream()
+    fb = torch.zeros((args.height, args.width), dtype=torch.int32, device=dev)
+
+    def frames(k):
+        r.accumID = 0
+        for _ in range(args.warmup):
+            r.render(fb, stream)
+        r.accumID = 0
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for _ in range(k):
+            r.render(fb, stream)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / k, r.accum().astype(np.float64)
+
+    K = args.steps
+    r.setNeural(None)
+    ms_exact, s_exact = frames(K)
+    for _ in range(K):
+        r.render(fb, stream)
+    torch.cuda.synchronize()
+    s_exact2 = r.accum().astype(np.float64)
+    c_exact = None
+    r.setNeural(f)
+    ms_net, s_net = frames(K)
+    r.accumID = 0
+    c_net = r.render_counted(fb, stream)
+    r.setNeural(None)
+    r.accumID = 0
+    c_exact = r.render_counted(fb, stream)
+    A, B, N = s_exact / K, (s_exact2 - s_exact) / K, s_net / K
+    rmse = float(np.sqrt(np.mean((N - A)[..., :3] ** 2)))
+    noise = float(np.sqrt(np.mean((B - A)[..., :3] ** 2)))
+    samples = args.width * args.height * args.spp
+    log(f"[bench] exact {ms_exact:.2f} ms/frame, neural {ms_net:.2f} ms/frame; image RMSE {rmse:.3g} "
+        f"(seed noise {noise:.3g}); element lookups exact {sum(c_exact[k] for k in ('tet', 'pyr', 'wedge', 'hex'))}, "
+        f"with network: {c_net['neural']} by the network + "
+        f"{sum(c_net[k] for k in ('tet', 'pyr', 'wedge', 'hex'))} exact")
+    return {
+        "metric": METRIC, "value": samples / (ms_net * 1e-3), "unit": "samples/s", "n_gpus": 1, "steps": K,
+        "warmup": args.warmup, "ms_per_step": ms_net, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "bf16 (network) / fp32", "data": "synthetic",
+        "config": {"workload": "1080p frame of the 3-level noise scene with the neural field answering the "
+                               "element stage of validated macrocells (hash grid 16 x 2^19 x 2, MLP 32-64-64-1)",
+                   "exact_samples_per_s": samples / (ms_exact * 1e-3), "exact_ms": ms_exact, "neural_ms": ms_net,
+                   "speedup_vs_exact": ms_exact / ms_net, "image_rmse_vs_exact_same_seeds": rmse,
+                   "exact_seed_noise_rmse": noise, "validation": v, "train_steps": args.train_steps,
+                   "train_batch": args.train_batch, "train_seconds": train_s, "pool": args.pool,
+                   "lr": [1e-2, args.lr_end],
+                   "final_loss": float(np.mean(loss[-50:])), "counters_neural": c_net,
+                   "counters_exact": c_exact},
+        "roofline": None, "cpu_baseline": None,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -268,8 +347,15 @@ def main():
     ap.add_argument("--rare-batch", type=int, default=8)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["frame", "batch"], default="frame",
-                    help="frame: the 1080p frame (default); batch: batched requests in one launch")
+    ap.add_argument("--workload", choices=["frame", "batch", "neural"], default="frame",
+                    help="frame: the 1080p frame (default); batch: batched requests in one launch; "
+                         "neural: exact vs neural-field element stage")
+    ap.add_argument("--pool", type=int, default=1 << 24)
+    ap.add_argument("--train-steps", type=int, default=2000)
+    ap.add_argument("--train-batch", type=int, default=1 << 18)
+    ap.add_argument("--val-points", type=int, default=64)
+    ap.add_argument("--lr-end", type=float, default=1e-4, help="learning rate at the last step (exponential decay)")
+    ap.add_argument("--val-bound", type=float, default=1e-3, help="validation bound, fraction of the value range")
     ap.add_argument("--requests", type=int, default=64)
     ap.add_argument("--batch-spp", type=int, default=16)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
@@ -291,6 +377,9 @@ def main():
     r, rgba = setup_renderer(model, args)
     if args.workload == "batch":
         print(json.dumps(batch_workload(r, args, dev)), flush=True)
+        return
+    if args.workload == "neural":
+        print(json.dumps(neural_workload(r, args, dev)), flush=True)
         return
     stream = torch.cuda.current_stream()
     fb = torch.zeros((args.height, args.width), dtype=torch.int32, device=dev)

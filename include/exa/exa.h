@@ -296,8 +296,9 @@ int exa_neural_set_params(exa_neural *n, const float *table, const float *W1, co
    elemFraction inside random stitching elements, the rest in random gridlets */
 int exa_neural_make_pool(exa_neural *n, exa_renderer *r, int64_t count, float elemFraction, uint32_t seed,
                          int64_t *numValid);
-/* Adam steps on random batches of the pool; lossOut[steps] (may be NULL) */
-int exa_neural_train(exa_neural *n, int steps, int batch, float lr, float *lossOut);
+/* Adam steps on random batches of the pool, learning rate decaying
+   exponentially from lr to lrEnd; lossOut[steps] (may be NULL) */
+int exa_neural_train(exa_neural *n, int steps, int batch, float lr, float lrEnd, float *lossOut);
 /* loss and gradients of one batch given on the host (x in [0,1]^3 over the
    model's cell bounds, value normalised to the value range; count % 64 == 0) */
 int exa_neural_gradients(exa_neural *n, const float *x01, const float *target, int count, float *loss,

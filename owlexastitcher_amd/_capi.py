@@ -169,7 +169,7 @@ _SIGS = {
     "exa_neural_get_params": (C.c_int, [P, P, P, P, P]),
     "exa_neural_set_params": (C.c_int, [P, P, P, P, P]),
     "exa_neural_make_pool": (C.c_int, [P, P, C.c_int64, F, C.c_uint32, P]),
-    "exa_neural_train": (C.c_int, [P, C.c_int, C.c_int, F, P]),
+    "exa_neural_train": (C.c_int, [P, C.c_int, C.c_int, F, F, P]),
     "exa_neural_gradients": (C.c_int, [P, P, P, C.c_int, P, P, P, P, P]),
     "exa_neural_apply_adam": (C.c_int, [P, F]),
     "exa_neural_evaluate": (C.c_int, [P, P, C.c_int64, P, P]),

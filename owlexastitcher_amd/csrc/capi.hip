@@ -1439,18 +1439,22 @@ static void neuralStep(exa_neural *n, const float4 *pool, int64_t poolSize, int 
   a.lossSum = lossSlot;
   neuralTrain(a, s);
 }
-// steps of Adam on batches drawn from the pool; lossOut (may be NULL) receives
-// the mean squared error of each step's batch (normalised values)
-int exa_neural_train(exa_neural *n, int steps, int batch, float lr, float *lossOut) {
+// steps of Adam on batches drawn from the pool, the learning rate decaying
+// exponentially from lr to lrEnd; lossOut (may be NULL) receives the mean
+// squared error of each step's batch (normalised values)
+int exa_neural_train(exa_neural *n, int steps, int batch, float lr, float lrEnd, float *lossOut) {
   return guarded([&] {
     require(n && steps >= 0 && batch >= 64 && batch % 64 == 0, "batch must be a positive multiple of 64");
+    require(lr > 0.f && lrEnd > 0.f, "learning rates must be positive");
     require(n->poolSize > 0, "no training pool (exa_neural_make_pool)");
     n->lossBuf.alloc(std::max(steps, 1));
     neuralZero(n->lossBuf.ptr, std::max(steps, 1), 0);
     for (int i = 0; i < steps; ++i) {
       n->step++;
+      // exponential decay from lr to lrEnd over the run
+      const float li = steps > 1 ? lr * powf(lrEnd / lr, (float)i / (float)(steps - 1)) : lr;
       neuralStep(n, n->pool.ptr, n->poolSize, batch, false, 0x85ebca6bu * (uint32_t)n->step, n->lossBuf.ptr + i, 0);
-      neuralAdam(n->master.ptr, n->grad.ptr, n->m.ptr, n->v.ptr, n->numParams, lr, 0.9f, 0.99f, 1e-10f, n->step,
+      neuralAdam(n->master.ptr, n->grad.ptr, n->m.ptr, n->v.ptr, n->numParams, li, 0.9f, 0.99f, 1e-10f, n->step,
                  0);
       n->pack(0);
     }

@@ -467,9 +467,12 @@ class NeuralField:
                                          C.byref(nv)))
         return nv.value
 
-    def train(self, steps, batch=1 << 18, lr=1e-2):
+    def train(self, steps, batch=1 << 18, lr=1e-2, lr_end=None):
+        """Adam; the learning rate decays exponentially from lr to lr_end
+        (default: constant).  Returns the loss of every step."""
         loss = np.zeros(max(int(steps), 1), np.float32)
-        check(lib().exa_neural_train(self.h, int(steps), int(batch), float(lr), _ptr(loss)))
+        check(lib().exa_neural_train(self.h, int(steps), int(batch), float(lr),
+                                     float(lr if lr_end is None else lr_end), _ptr(loss)))
         return loss[:int(steps)]
 
     def gradients(self, x01, target):
