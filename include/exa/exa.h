@@ -63,6 +63,17 @@ int exa_amr_create_gaussians(int baseRes, int coarseLevel, int fineLo, int fineH
                              uint64_t seed, exa_amr **out);
 int exa_amr_create_noise(int baseRes, int baseLevel, float thresh1, float thresh2, int octaves,
                          float contrast, uint64_t seed, exa_amr **out);
+/* Expanding spherical shock shell with turbulence in its wake (the large
+   configuration: baseRes 256, baseLevel 4, radius 1200, thickness 40 over
+   [0,4096]^3): shell at level 1, level-0 patches where the turbulence exceeds
+   thetaCore, level >= 2 in the wake where it exceeds thetaWake; leaf levels
+   follow a graded distance field, so touching leaves differ by at most one
+   level.  Scalar = shock profile x turbulence + wake turbulence, in [0,1]. */
+int exa_amr_create_shock(int baseRes, int baseLevel, float radius, float thickness, float thetaCore,
+                         float thetaWake, uint64_t seed, exa_amr **out);
+/* leaf cells per level (counts[16]) of that scene, without building it */
+int exa_amr_count_shock(int baseRes, int baseLevel, float radius, float thickness, float thetaCore,
+                        float thetaWake, uint64_t seed, int64_t counts[16]);
 int exa_amr_create_from_cells(const exa_amr_cell *cells, const float *scalars, int64_t n, exa_amr **out);
 int64_t exa_amr_num_cells(const exa_amr *a);
 int exa_amr_get(const exa_amr *a, exa_amr_cell *cells, float *scalars);

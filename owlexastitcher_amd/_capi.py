@@ -83,6 +83,8 @@ _SIGS = {
     "exa_version": (C.c_int, []),
     "exa_amr_create_gaussians": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64, P]),
     "exa_amr_create_noise": (C.c_int, [C.c_int, C.c_int, F, F, C.c_int, F, C.c_uint64, P]),
+    "exa_amr_create_shock": (C.c_int, [C.c_int, C.c_int, F, F, F, F, C.c_uint64, P]),
+    "exa_amr_count_shock": (C.c_int, [C.c_int, C.c_int, F, F, F, F, C.c_uint64, P]),
     "exa_amr_create_from_cells": (C.c_int, [P, P, I64, P]),
     "exa_amr_num_cells": (I64, [P]),
     "exa_amr_get": (C.c_int, [P, P, P]),

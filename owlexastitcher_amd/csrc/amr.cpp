@@ -233,4 +233,147 @@ void amrNoise(AmrScene &out, int baseRes, int baseLevel, float thresh1, float th
   out.params = {(float)baseRes, (float)baseLevel, thresh1, thresh2, (float)octaves, contrast};
 }
 
+// ---------------------------------------------------------------------------
+// Expanding spherical shock shell with turbulence in its wake.
+//
+// The leaf level of a point follows a graded distance field
+//   T(x) = floor(log2(max(1, D(x) / G))),  clamped to [0, baseLevel],
+// D = min(D_shell, D_core, D_wake), with ds the distance to a noise-perturbed
+// sphere minus the shell half thickness and n a low-frequency turbulence:
+//   D_shell = 2.5G + k ds                          (shell band at level 1),
+//   D_core  = G + k ds + S max(0, thetaCore - n)   (level-0 patches in it),
+//   D_wake  = 4.75G + S max(0, thetaWake - n) + max(0, r - R)
+//                                                  (level-2 wake inside);
+// the offsets above 2G and 4G leave room for the split test's margin.
+// A cell of level L is split when a lower bound of D over the cell (D at the
+// centre minus Lip * half diagonal) is below G 2^L.  For a field with
+// Lipschitz constant Lip and G > 6.1 Lip, touching leaves differ by at most
+// one level (vertex 2:1 balance, which the dual mesh needs).  With value
+// noise of slope at most 2.6 f per octave, k = 1.5, S = 400 (per 4096 of
+// domain) and the noise frequencies below bound Lip by 2.5 (G = 16).  The scalar field uses a finer
+// turbulence than the refinement.
+// ---------------------------------------------------------------------------
+namespace {
+struct ShockField {
+  double cx, cy, cz, R, h, G, lip, k, slope, wobble;
+  Fbm shellNoise, refine, turb;
+  double thetaCore, thetaWake;
+  double dShell(double x, double y, double z, double &r) const {
+    const double dx = x - cx, dy = y - cy, dz = z - cz;
+    r = sqrt(dx * dx + dy * dy + dz * dz);
+    const double Rn = R + wobble * (shellNoise(x, y, z) - 0.5);
+    return std::max(0.0, fabs(r - Rn) - h);
+  }
+  double D(double x, double y, double z) const {
+    double r;
+    const double ds = dShell(x, y, z, r);
+    const double n = refine(x, y, z);
+    const double shell = 2.5 * G + k * ds;
+    const double core = G + k * ds + slope * std::max(0.0, thetaCore - n);
+    const double wake = 4.75 * G + slope * std::max(0.0, thetaWake - n) + std::max(0.0, r - R);
+    return std::min(std::min(shell, core), wake);
+  }
+  double scalar(double x, double y, double z) const {
+    double r;
+    const double ds = dShell(x, y, z, r);
+    const double t = turb(x, y, z);
+    const double shock = exp(-(ds / (2.0 * h)) * (ds / (2.0 * h)));
+    const double wake = r < R ? 0.35 * t * (r / R) : 0.05 * t;
+    return std::min(1.0, 0.8 * shock * (0.75 + 0.5 * t) + wake);
+  }
+};
+
+ShockField makeShock(double domain, float radius, float thickness, float thetaCore, float thetaWake,
+                     uint64_t seed) {
+  ShockField F;
+  F.cx = F.cy = F.cz = 0.5 * domain;
+  F.R = radius;
+  F.h = 0.5 * thickness;
+  F.G = 16.0;
+  F.lip = 2.5;
+  F.k = 1.5;
+  // amplitudes scale with the domain so that the slopes stay domain-free
+  F.slope = 400.0 * domain / 4096.0;
+  F.wobble = 60.0 * domain / 4096.0;
+  F.shellNoise = Fbm{2, 3.0 / domain, 1.0, seed};
+  F.refine = Fbm{2, 2.0 / domain, 1.0, seed + 5};
+  F.turb = Fbm{3, 6.0 / domain, 1.0, seed + 7};
+  F.thetaCore = thetaCore;
+  F.thetaWake = thetaWake;
+  return F;
+}
+
+template <typename Emit>
+void shockCell(const ShockField &F, int x, int y, int z, int L, Emit &&emit) {
+  const double s = (double)(1 << L);
+  const double dLow = F.D(x + 0.5 * s, y + 0.5 * s, z + 0.5 * s) - F.lip * 0.8660254037844386 * s;
+  if (L == 0 || dLow >= F.G * s) {
+    emit(x, y, z, L);
+    return;
+  }
+  const int hs = 1 << (L - 1);
+  for (int dz = 0; dz < 2; ++dz)
+    for (int dy = 0; dy < 2; ++dy)
+      for (int dx = 0; dx < 2; ++dx) shockCell(F, x + dx * hs, y + dy * hs, z + dz * hs, L - 1, emit);
+}
+}  // namespace
+
+void amrShockCount(int baseRes, int baseLevel, float radius, float thickness, float thetaCore, float thetaWake,
+                   uint64_t seed, int64_t counts[16]) {
+  if (baseRes <= 0 || baseLevel < 1 || baseLevel > 15 || radius <= 0.f || thickness <= 0.f)
+    throw std::runtime_error("amrShock: invalid parameters");
+  const int64_t B = baseRes;
+  const int W = 1 << baseLevel;
+  const ShockField F = makeShock((double)B * W, radius, thickness, thetaCore, thetaWake, seed);
+  std::vector<std::atomic<int64_t>> c(16);
+  for (auto &v : c) v.store(0);
+  parallelFor(B * B * B, 4096, [&](int64_t b, int64_t e) {
+    int64_t local[16] = {};
+    for (int64_t id = b; id < e; ++id)
+      shockCell(F, (int)(id % B) * W, (int)((id / B) % B) * W, (int)(id / (B * B)) * W, baseLevel,
+                [&](int, int, int, int L) { local[L]++; });
+    for (int l = 0; l < 16; ++l) c[l] += local[l];
+  });
+  for (int l = 0; l < 16; ++l) counts[l] = c[l].load();
+}
+
+void amrShock(AmrScene &out, int baseRes, int baseLevel, float radius, float thickness, float thetaCore,
+              float thetaWake, uint64_t seed) {
+  if (baseRes <= 0 || baseLevel < 1 || baseLevel > 12 || radius <= 0.f || thickness <= 0.f)
+    throw std::runtime_error("amrShock: invalid parameters");
+  const int64_t B = baseRes;
+  const int W = 1 << baseLevel;
+  const ShockField F = makeShock((double)B * W, radius, thickness, thetaCore, thetaWake, seed);
+  // base cells in z-major order, chunks of base cells in parallel
+  const int64_t nb = B * B * B;
+  const int64_t chunk = 4096;
+  const int64_t nchunks = (nb + chunk - 1) / chunk;
+  std::vector<std::vector<AmrCell>> parts(nchunks);
+  parallelFor(nchunks, 1, [&](int64_t b, int64_t e) {
+    for (int64_t ci = b; ci < e; ++ci) {
+      std::vector<AmrCell> &v = parts[ci];
+      for (int64_t id = ci * chunk; id < std::min(nb, (ci + 1) * chunk); ++id)
+        shockCell(F, (int)(id % B) * W, (int)((id / B) % B) * W, (int)(id / (B * B)) * W, baseLevel,
+                  [&](int x, int y, int z, int L) { v.push_back(AmrCell{{x, y, z}, L}); });
+    }
+  });
+  size_t total = 0;
+  for (auto &v : parts) total += v.size();
+  out.cells.clear();
+  out.cells.reserve(total);
+  for (auto &v : parts) {
+    out.cells.insert(out.cells.end(), v.begin(), v.end());
+    std::vector<AmrCell>().swap(v);
+  }
+  out.scalars.resize(out.cells.size());
+  parallelFor((int64_t)out.cells.size(), 1 << 14, [&](int64_t b, int64_t e) {
+    for (int64_t i = b; i < e; ++i) {
+      const AmrCell &c = out.cells[i];
+      const double hw = 0.5 * (double)(1 << c.level);
+      out.scalars[i] = (float)F.scalar(c.pos[0] + hw, c.pos[1] + hw, c.pos[2] + hw);
+    }
+  });
+  out.params = {(float)baseRes, (float)baseLevel, radius, thickness, thetaCore, thetaWake};
+}
+
 }  // namespace exa

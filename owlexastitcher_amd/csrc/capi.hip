@@ -511,6 +511,22 @@ int exa_amr_create_noise(int baseRes, int baseLevel, float t1, float t2, int oct
     *out = a.release();
   });
 }
+int exa_amr_create_shock(int baseRes, int baseLevel, float radius, float thickness, float thetaCore,
+                         float thetaWake, uint64_t seed, exa_amr **out) {
+  return guarded([&] {
+    require(out, "out is NULL");
+    auto a = std::make_unique<exa_amr>();
+    amrShock(a->scene, baseRes, baseLevel, radius, thickness, thetaCore, thetaWake, seed);
+    *out = a.release();
+  });
+}
+int exa_amr_count_shock(int baseRes, int baseLevel, float radius, float thickness, float thetaCore,
+                        float thetaWake, uint64_t seed, int64_t counts[16]) {
+  return guarded([&] {
+    require(counts, "counts is NULL");
+    amrShockCount(baseRes, baseLevel, radius, thickness, thetaCore, thetaWake, seed, counts);
+  });
+}
 int exa_amr_create_from_cells(const exa_amr_cell *cells, const float *scalars, int64_t n, exa_amr **out) {
   return guarded([&] {
     require(out && cells && scalars && n > 0, "invalid cells");

@@ -35,6 +35,16 @@ void amrGaussians(AmrScene &out, int baseRes, int coarseLevel, int fineLo, int f
 void amrNoise(AmrScene &out, int baseRes, int baseLevel, float thresh1, float thresh2,
               int octaves, float contrast, uint64_t seed);
 
+// Multi-level expanding-shock scene: base cells of width 2^baseLevel, the
+// shell (radius, thickness) refined to level 1 with level-0 patches where the
+// turbulence exceeds thetaCore, level >= 2 in the wake where it exceeds
+// thetaWake; graded so that touching leaves differ by at most one level.
+void amrShock(AmrScene &out, int baseRes, int baseLevel, float radius, float thickness, float thetaCore,
+              float thetaWake, uint64_t seed);
+// leaf counts per level of the same scene, without building it
+void amrShockCount(int baseRes, int baseLevel, float radius, float thickness, float thetaCore, float thetaWake,
+                   uint64_t seed, int64_t counts[16]);
+
 // ---------------------------------------------------------------------------
 // dual mesh (tools/amrMakeDualMesh/makeDual.cpp, makeGrids.cpp)
 // ---------------------------------------------------------------------------

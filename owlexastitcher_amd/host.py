@@ -55,6 +55,25 @@ class AmrScene:
         return cls(h)
 
     @classmethod
+    def shock(cls, base_res=256, base_level=4, radius=1200.0, thickness=40.0, theta_core=0.74, theta_wake=0.55,
+              seed=1):
+        """Expanding spherical shock shell with turbulence in its wake (5 levels
+        at the defaults, over [0, base_res * 2^base_level]^3)."""
+        h = C.c_void_p()
+        check(lib().exa_amr_create_shock(int(base_res), int(base_level), float(radius), float(thickness),
+                                         float(theta_core), float(theta_wake), int(seed), C.byref(h)))
+        return cls(h)
+
+    @staticmethod
+    def shock_counts(base_res=256, base_level=4, radius=1200.0, thickness=40.0, theta_core=0.74, theta_wake=0.55,
+                     seed=1):
+        """Leaf cells per level of AmrScene.shock(...) without building it."""
+        c = np.zeros(16, np.int64)
+        check(lib().exa_amr_count_shock(int(base_res), int(base_level), float(radius), float(thickness),
+                                        float(theta_core), float(theta_wake), int(seed), _ptr(c)))
+        return c[:base_level + 1]
+
+    @classmethod
     def from_cells(cls, cells, scalars):
         cells = np.ascontiguousarray(cells, dtype=np.int32).reshape(-1, 4)
         scalars = _f32(scalars)
