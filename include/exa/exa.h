@@ -91,8 +91,42 @@ int exa_dual_mesh_get(const exa_dual_mesh *d, int32_t *indices, float *vertices,
                       float *gridletScalars);
 void exa_dual_mesh_destroy(exa_dual_mesh *d);
 
-/* ---- model: exa::ExaStitchModel (ExaStitchModel.cpp:24-257) ---- */
+/* ---- on-disk formats ----
+   .cells: 16-byte {int32 x, y, z, level} records (makeDual.cpp:932-940);
+   .scalars: raw float32 (ExaStitchModel.cpp:44-54); _<L>.cubes: 48-byte
+   {float lower[3]; int32 level; int32 scalarIDs[8]} (makeDual.cpp:445-449,
+   893-905); .grids: {int32 lower[3], level, dims[3], scalarIDs[(dims+1)^3]}
+   per gridlet (makeGrids.cpp:240-246); .xf: headless.cpp:258-283.  The
+   element mesh (the reference's umesh file) uses this library's EXAELEM1
+   format: vertices, per-vertex scalar IDs and/or values, tets, pyramids,
+   wedges, hexes. */
+/* reads a .cells file and its .scalars (one per cell) */
+int exa_amr_load(const char *cellsPath, const char *scalarsPath, exa_amr **out);
+int exa_amr_save(const exa_amr *a, const char *cellsPath, const char *scalarsPath);
+/* amrMakeDualMesh + amrMakeGrids to files (makeDual.cpp:908-960,
+   makeGrids.cpp:262-343): the element mesh (with scalar IDs and values per
+   vertex), optionally <cubesPrefix>_<L>.cubes and the .grids file; NULL
+   paths are skipped.  counts[11] as exa_dual_mesh_counts (may be NULL). */
+int exa_amr_write_dual_mesh(const exa_amr *a, const char *elementMeshPath, const char *cubesPrefix,
+                            const char *gridsPath, int64_t counts[11]);
+/* amrMakeGrids (makeGrids.cpp:262-343): bricks of each cubes file (level
+   from its "_<L>.cubes" suffix), appended to gridsPath in argument order;
+   *conflicts (may be NULL) counts inconsistent brick-vertex writes */
+int exa_make_grids(const char *const *cubesPaths, int numPaths, const char *gridsPath, int64_t *conflicts);
+/* ExaStitchModel::load(umesh, grids, scalars) (ExaStitchModel.cpp:24-257):
+   any path may be NULL or empty.  Vertex value = scalars[vertexTag] when
+   both exist, else the stored per-vertex value; unreferenced vertices are
+   compacted when a grids file is given; a gridlet scalar ID beyond the
+   scalars file is a hole (NaN).  Unlike the reference, a path that cannot
+   be read, a truncated record or a vertex tag beyond the scalars is an
+   error. */
 typedef struct exa_model exa_model;
+int exa_model_load(const char *elementMeshPath, const char *gridsPath, const char *scalarsPath, exa_model **out);
+/* writes a .xf file (read with exa_xf_load): params = opacityScale,
+   absDomain lo/hi, relDomain lo/hi */
+int exa_xf_write(const char *path, const float params[5], const float *rgba, int numColors);
+
+/* ---- model: exa::ExaStitchModel (ExaStitchModel.cpp:24-257) ---- */
 typedef struct {
   const float *vertices; /* float4: xyz + scalar */
   int64_t numVertices;
@@ -108,6 +142,10 @@ int exa_model_create_from_dual_mesh(const exa_dual_mesh *d, exa_model **out);
 /* cellBounds (lo xyz, hi xyz) and valueRange (Model.h:42-43) */
 int exa_model_get_bounds(const exa_model *m, float cellBounds[6], float valueRange[2]);
 int exa_model_get_counts(const exa_model *m, int64_t counts[4]); /* vertices, elements, gridlets, scalars */
+/* copies the model buffers (ExaStitchModel.h members vertices, indices,
+   gridlets, gridletScalars), sized by exa_model_get_counts; NULL skips one */
+int exa_model_get(const exa_model *m, float *vertices, int32_t *indices, exa_gridlet *gridlets,
+                  float *gridletScalars);
 int exa_model_set_num_grid_cells(exa_model *m, const int32_t numMCs[3]);                     /* Model.cpp:25-33 */
 int exa_model_set_voxel_space_transform(exa_model *m, const float from[6], const float to[6]); /* Model.cpp:35-48 */
 void exa_model_destroy(exa_model *m);

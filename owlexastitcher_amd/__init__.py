@@ -9,5 +9,5 @@ from ._capi import LIB_PATH, ExaError, lib  # noqa: F401
 from .host import (ACCEL_BVH, ACCEL_TABLES, DIRECT_LIGHTING, PATH_TRACER, RENDER_MEGAKERNEL,  # noqa: F401
                    RENDER_PERSISTENT, AmrScene, DualMesh, ExaStitchModel, ExaStitchSampler, NeuralField, OWLRenderer,
                    RenderBatch,
-                   camera_look_at,
-                   tf_rasterize, xf_load)
+                   camera_look_at, make_grids,
+                   tf_rasterize, xf_load, xf_save)

@@ -86,6 +86,13 @@ _SIGS = {
     "exa_amr_create_shock": (C.c_int, [C.c_int, C.c_int, F, F, F, F, C.c_uint64, P]),
     "exa_amr_count_shock": (C.c_int, [C.c_int, C.c_int, F, F, F, F, C.c_uint64, P]),
     "exa_amr_create_from_cells": (C.c_int, [P, P, I64, P]),
+    "exa_amr_load": (C.c_int, [C.c_char_p, C.c_char_p, P]),
+    "exa_amr_save": (C.c_int, [P, C.c_char_p, C.c_char_p]),
+    "exa_amr_write_dual_mesh": (C.c_int, [P, C.c_char_p, C.c_char_p, C.c_char_p, P]),
+    "exa_make_grids": (C.c_int, [P, C.c_int, C.c_char_p, P]),
+    "exa_model_load": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, P]),
+    "exa_model_get": (C.c_int, [P, P, P, P, P]),
+    "exa_xf_write": (C.c_int, [C.c_char_p, P, P, C.c_int]),
     "exa_amr_num_cells": (I64, [P]),
     "exa_amr_get": (C.c_int, [P, P, P]),
     "exa_amr_get_params": (C.c_int, [P, P, C.c_int]),

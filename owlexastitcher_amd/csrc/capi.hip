@@ -12,6 +12,7 @@
 #include <vector>
 #include "../../include/exa/exa.h"
 #include "exa_kernels.h"
+#include "exa_io.h"
 #include "neural_kernels.h"
 
 namespace exa {
@@ -589,6 +590,93 @@ int exa_dual_mesh_get(const exa_dual_mesh *d, int32_t *indices, float *vertices,
 }
 void exa_dual_mesh_destroy(exa_dual_mesh *d) { delete d; }
 
+// ---- on-disk formats ----------------------------------------------------------
+static std::string pathOf(const char *p) { return p ? std::string(p) : std::string(); }
+
+int exa_amr_load(const char *cellsPath, const char *scalarsPath, exa_amr **out) {
+  return guarded([&] {
+    require(out && cellsPath && scalarsPath, "invalid arguments");
+    auto a = std::make_unique<exa_amr>();
+    readCells(cellsPath, a->scene.cells);
+    readScalars(scalarsPath, a->scene.scalars);
+    require(!a->scene.cells.empty(), "no cells in the cells file");
+    require(a->scene.scalars.size() >= a->scene.cells.size(), "fewer scalars than cells");
+    a->scene.scalars.resize(a->scene.cells.size());
+    *out = a.release();
+  });
+}
+int exa_amr_save(const exa_amr *a, const char *cellsPath, const char *scalarsPath) {
+  return guarded([&] {
+    require(a, "amr is NULL");
+    if (cellsPath) writeCells(cellsPath, a->scene.cells);
+    if (scalarsPath) writeScalars(scalarsPath, a->scene.scalars);
+  });
+}
+int exa_amr_write_dual_mesh(const exa_amr *a, const char *elementMeshPath, const char *cubesPrefix,
+                            const char *gridsPath, int64_t counts[11]) {
+  return guarded([&] {
+    require(a, "amr is NULL");
+    exa_dual_mesh d;
+    buildDualMesh(a->scene.cells.data(), a->scene.scalars.data(), (int64_t)a->scene.cells.size(), d.mesh, true);
+    const DualMesh &m = d.mesh;
+    if (elementMeshPath) {
+      ElementMesh em;
+      const int64_t nv = (int64_t)m.vertices.size() / 4;
+      em.xyz.resize(3 * nv);
+      em.values.resize(nv);
+      for (int64_t i = 0; i < nv; ++i) {
+        for (int k = 0; k < 3; ++k) em.xyz[3 * i + k] = m.vertices[4 * i + k];
+        em.values[i] = m.vertices[4 * i + 3];
+      }
+      em.vertexTag = m.vertexTags;
+      static const int nvs[4] = {4, 5, 6, 8};
+      int64_t e = 0;
+      for (int t = 0; t < 4; ++t)
+        for (int64_t i = 0; i < m.counts[t]; ++i, ++e)
+          for (int j = 0; j < nvs[t]; ++j) em.elems[t].push_back(m.indices[8 * e + j]);
+      writeElementMesh(elementMeshPath, em);
+    }
+    if (cubesPrefix) writeCubes(cubesPrefix, m.cubes);
+    if (gridsPath) writeGrids(gridsPath, m.gridlets, m.gridletScalarIDs, false);
+    if (counts) {
+      exa_dual_mesh_counts(&d, counts);
+    }
+  });
+}
+int exa_make_grids(const char *const *cubesPaths, int numPaths, const char *gridsPath, int64_t *conflicts) {
+  return guarded([&] {
+    require(gridsPath && numPaths >= 0 && (numPaths == 0 || cubesPaths), "invalid arguments");
+    std::vector<std::string> paths;
+    for (int i = 0; i < numPaths; ++i) {
+      require(cubesPaths[i], "NULL cubes path");
+      paths.push_back(cubesPaths[i]);
+    }
+    makeGridsFiles(paths, gridsPath, conflicts);
+  });
+}
+int exa_model_load(const char *elementMeshPath, const char *gridsPath, const char *scalarsPath, exa_model **out) {
+  return guarded([&] {
+    require(out, "invalid arguments");
+    auto m = std::make_unique<exa_model>();
+    loadModelFiles(pathOf(elementMeshPath), pathOf(gridsPath), pathOf(scalarsPath), m->m);
+    require(m->m.numElements() > 0 || !m->m.gridlets.empty(), "the model files hold no elements and no gridlets");
+    *out = m.release();
+  });
+}
+int exa_xf_write(const char *path, const float params[5], const float *rgba, int numColors) {
+  return guarded([&] {
+    require(path && params && numColors >= 0 && (numColors == 0 || rgba), "invalid arguments");
+    XfFile xf;
+    xf.opacityScale = params[0];
+    xf.absDomain[0] = params[1];
+    xf.absDomain[1] = params[2];
+    xf.relDomain[0] = params[3];
+    xf.relDomain[1] = params[4];
+    xf.colorMap.assign(rgba, rgba + 4 * (size_t)numColors);
+    writeXf(path, xf);
+  });
+}
+
 // ---- model -----------------------------------------------------------------
 int exa_model_create(const exa_stitch_model_desc *desc, exa_model **out) {
   return guarded([&] {
@@ -635,6 +723,17 @@ int exa_model_get_counts(const exa_model *m, int64_t c[4]) {
     c[1] = m->m.numElements();
     c[2] = (int64_t)m->m.gridlets.size();
     c[3] = (int64_t)m->m.gridletScalars.size();
+  });
+}
+int exa_model_get(const exa_model *m, float *vertices, int32_t *indices, exa_gridlet *gridlets,
+                  float *gridletScalars) {
+  return guarded([&] {
+    require(m, "model is NULL");
+    const HostModel &h = m->m;
+    if (vertices) memcpy(vertices, h.vertices.data(), h.vertices.size() * sizeof(float));
+    if (indices) memcpy(indices, h.indices.data(), h.indices.size() * sizeof(int32_t));
+    if (gridlets) memcpy(gridlets, h.gridlets.data(), h.gridlets.size() * sizeof(Gridlet));
+    if (gridletScalars) memcpy(gridletScalars, h.gridletScalars.data(), h.gridletScalars.size() * sizeof(float));
   });
 }
 int exa_model_set_num_grid_cells(exa_model *m, const int32_t numMCs[3]) {
@@ -1702,28 +1801,16 @@ int exa_xf_load(const char *path, float *opacityScale, float absDomain[2], float
                 float *rgba, int maxColors) {
   return guarded([&] {
     require(path, "path is NULL");
-    std::ifstream in(path, std::ios::binary);
-    if (!in.good()) throw Status(EXA_ERR_INVALID, std::string("cannot open ") + path);
-    uint64_t magic = 0;
-    in.read((char *)&magic, 8);
-    if (magic != 0x1235abc000ull) throw Status(EXA_ERR_INVALID, std::string(path) + ": not a .xf file");
-    float os, ad[2], rd[2];
-    int32_t n = 0;
-    in.read((char *)&os, 4);
-    in.read((char *)ad, 8);
-    in.read((char *)rd, 8);
-    in.read((char *)&n, 4);
-    if (!in.good() || n <= 0 || n > (1 << 20)) throw Status(EXA_ERR_INVALID, "truncated .xf header");
-    std::vector<float> cm(4 * (size_t)n);
-    in.read((char *)cm.data(), cm.size() * 4);
-    if (!in.good()) throw Status(EXA_ERR_INVALID, "truncated .xf color map");
-    if (opacityScale) *opacityScale = os;
-    if (absDomain) memcpy(absDomain, ad, 8);
-    if (relDomain) memcpy(relDomain, rd, 8);
+    XfFile xf;
+    readXf(path, xf);
+    const int n = (int)(xf.colorMap.size() / 4);
+    if (opacityScale) *opacityScale = xf.opacityScale;
+    if (absDomain) memcpy(absDomain, xf.absDomain, 8);
+    if (relDomain) memcpy(relDomain, xf.relDomain, 8);
     if (numColors) *numColors = n;
     if (rgba) {
       require(maxColors >= n, "rgba buffer too small");
-      memcpy(rgba, cm.data(), cm.size() * 4);
+      memcpy(rgba, xf.colorMap.data(), xf.colorMap.size() * 4);
     }
   });
 }

@@ -125,12 +125,6 @@ struct CellHash {
   }
 };
 
-struct CubeRec {
-  int32_t level;
-  int32_t cid[3];
-  int32_t scalarIDs[8];
-};
-
 struct ChunkOut {
   std::vector<int64_t> elem[4];  // flattened sorted-cell indices per type
   std::vector<CubeRec> cubes;
@@ -291,7 +285,87 @@ void doCell(const Exa &exa, int64_t ci, ChunkOut &out) {
 
 }  // namespace
 
-void buildDualMesh(const AmrCell *in, const float *scalars, int64_t n, DualMesh &out) {
+// makeGrids.cpp:46-209: bricks of <= 8^3 same-level cubes per macrocell
+// (cid / 8, rounded down), each brick the bounding box of its cubes; a brick
+// vertex takes the scalar ID of the cube corner there (vtkOrder), -1 where no
+// cube reaches.  Bricks come in (level, mc.x, mc.y, mc.z) order, the order of
+// makeGrids's per-level std::map when levels ascend.
+void makeBricks(const std::vector<CubeRec> &cubes, std::vector<Gridlet> &gridlets, const BrickSink &sink,
+                int64_t &conflicts) {
+  const int64_t numCubes = (int64_t)cubes.size();
+  auto mcOf = [](int32_t c) { return (c < 0 ? c - 7 : c) / 8; };  // makeGrids.cpp:79-86
+  std::vector<uint64_t> ckeys(numCubes);
+  std::vector<int64_t> corder(numCubes);
+  for (int64_t i = 0; i < numCubes; ++i) {
+    const CubeRec &c = cubes[i];
+    if (c.level < 0 || c.level > 15) throw std::runtime_error("makeBricks: cube level out of range [0,15]");
+    uint64_t key = (uint64_t)c.level << 60;
+    for (int k = 0; k < 3; ++k) {
+      if (c.cid[k] < -(1 << 22) || c.cid[k] >= (1 << 22))
+        throw std::runtime_error("makeBricks: cube position out of range");
+      key |= (uint64_t)(mcOf(c.cid[k]) + (1 << 19)) << (40 - 20 * k);
+    }
+    ckeys[i] = key;
+    corder[i] = i;
+  }
+  radixSort(ckeys, corder);
+  std::vector<int64_t> brickStart;
+  for (int64_t i = 0; i < numCubes; ++i)
+    if (i == 0 || ckeys[i] != ckeys[i - 1]) brickStart.push_back(i);
+  brickStart.push_back(numCubes);
+  ckeys.clear();
+  ckeys.shrink_to_fit();
+  const int64_t numBricks = (int64_t)brickStart.size() - 1;
+  gridlets.resize(numBricks);
+  std::vector<int64_t> scalarStart(numBricks + 1, 0);
+  for (int64_t b = 0; b < numBricks; ++b) {
+    int32_t lo[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, hi[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
+    for (int64_t i = brickStart[b]; i < brickStart[b + 1]; ++i) {
+      const CubeRec &c = cubes[corder[i]];
+      for (int k = 0; k < 3; ++k) {
+        lo[k] = std::min(lo[k], c.cid[k]);
+        hi[k] = std::max(hi[k], c.cid[k] + 1);
+      }
+    }
+    Gridlet &g = gridlets[b];
+    for (int k = 0; k < 3; ++k) {
+      g.lower[k] = lo[k];
+      g.dims[k] = hi[k] - lo[k];
+    }
+    g.level = cubes[corder[brickStart[b]]].level;
+    scalarStart[b + 1] = scalarStart[b] + (int64_t)(g.dims[0] + 1) * (g.dims[1] + 1) * (g.dims[2] + 1);
+    if (scalarStart[b + 1] >= ((int64_t)1 << 32))
+      throw std::runtime_error("makeBricks: gridlet scalars exceed 32-bit offsets");
+    g.begin = (uint32_t)scalarStart[b];
+  }
+  sink.allocate(scalarStart[numBricks]);
+  std::atomic<int64_t> bad{0};
+  parallelChunks(numBricks, [&](int64_t b) {
+    const Gridlet &g = gridlets[b];
+    const int nx = g.dims[0], ny = g.dims[1];
+    const int64_t ns = scalarStart[b + 1] - scalarStart[b];
+    std::vector<int32_t> ids(ns, -1);
+    static const int vtkOrder[8] = {0, 1, 3, 2, 4, 5, 7, 6};
+    int64_t nbad = 0;
+    for (int64_t i = brickStart[b]; i < brickStart[b + 1]; ++i) {
+      const CubeRec &c = cubes[corder[i]];
+      const int bx = c.cid[0] - g.lower[0], by = c.cid[1] - g.lower[1], bz = c.cid[2] - g.lower[2];
+      for (int iz = 0; iz < 2; iz++)
+        for (int iy = 0; iy < 2; iy++)
+          for (int ix = 0; ix < 2; ix++) {
+            const int64_t idx = (bx + ix) + (int64_t)(nx + 1) * ((by + iy) + (int64_t)(ny + 1) * (bz + iz));
+            const int32_t sid = c.scalarIDs[vtkOrder[4 * iz + 2 * iy + ix]];
+            if (ids[idx] != -1 && ids[idx] != sid) nbad++;
+            ids[idx] = sid;
+          }
+    }
+    sink.brick(b, scalarStart[b], ids.data(), ns);
+    bad += nbad;
+  });
+  conflicts += bad.load();
+}
+
+void buildDualMesh(const AmrCell *in, const float *scalars, int64_t n, DualMesh &out, bool keepIds) {
   if (n <= 0) throw std::runtime_error("buildDualMesh: no cells");
   if (n >= (int64_t)1 << 31) throw std::runtime_error("buildDualMesh: too many cells");
   Exa exa;
@@ -357,9 +431,11 @@ void buildDualMesh(const AmrCell *in, const float *scalars, int64_t n, DualMesh 
   for (int64_t i = 0; i < n; ++i)
     if (vid[i] >= 0) vid[i] = (int32_t)numVerts++;
   out.vertices.assign(4 * numVerts, 0.f);
+  if (keepIds) out.vertexTags.assign(numVerts, -1);
   for (int64_t i = 0; i < n; ++i)
     if (vid[i] >= 0) {
       const SortedCell &c = exa.cells[i];
+      if (keepIds) out.vertexTags[vid[i]] = c.scalarID;
       float *v = &out.vertices[4 * (int64_t)vid[i]];
       for (int k = 0; k < 3; ++k) v[k] = (float)c.pos[k] + 0.5f * (float)(1 << c.level);
       v[3] = scalars[c.scalarID];
@@ -392,73 +468,21 @@ void buildDualMesh(const AmrCell *in, const float *scalars, int64_t n, DualMesh 
   }
   outs.clear();
   out.numCubes = (int64_t)cubes.size();
-  // gridlets: bricks of <= 8^3 cubes per level and macrocell
-  // (makeGrids.cpp:46,79-93,96-209)
-  const int64_t numCubes = (int64_t)cubes.size();
-  std::vector<uint64_t> ckeys(numCubes);
-  std::vector<int64_t> corder(numCubes);
-  for (int64_t i = 0; i < numCubes; ++i) {
-    const CubeRec &c = cubes[i];
-    ckeys[i] = ((uint64_t)c.level << 60) | ((uint64_t)(c.cid[0] >> 3) << 40) |
-               ((uint64_t)(c.cid[1] >> 3) << 20) | (uint64_t)(c.cid[2] >> 3);
-    corder[i] = i;
-  }
-  radixSort(ckeys, corder);
-  std::vector<int64_t> brickStart;
-  for (int64_t i = 0; i < numCubes; ++i)
-    if (i == 0 || ckeys[i] != ckeys[i - 1]) brickStart.push_back(i);
-  brickStart.push_back(numCubes);
-  const int64_t numBricks = (int64_t)brickStart.size() - 1;
-  out.gridlets.resize(numBricks);
-  std::vector<int64_t> scalarStart(numBricks + 1, 0);
-  for (int64_t b = 0; b < numBricks; ++b) {
-    int32_t lo[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, hi[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
-    for (int64_t i = brickStart[b]; i < brickStart[b + 1]; ++i) {
-      const CubeRec &c = cubes[corder[i]];
-      for (int k = 0; k < 3; ++k) {
-        lo[k] = std::min(lo[k], c.cid[k]);
-        hi[k] = std::max(hi[k], c.cid[k] + 1);
-      }
-    }
-    Gridlet &g = out.gridlets[b];
-    for (int k = 0; k < 3; ++k) {
-      g.lower[k] = lo[k];
-      g.dims[k] = hi[k] - lo[k];
-    }
-    g.level = cubes[corder[brickStart[b]]].level;
-    scalarStart[b + 1] =
-        scalarStart[b] + (int64_t)(g.dims[0] + 1) * (g.dims[1] + 1) * (g.dims[2] + 1);
-    if (scalarStart[b + 1] >= ((int64_t)1 << 32))
-      throw std::runtime_error("buildDualMesh: gridlet scalars exceed 32-bit offsets");
-    g.begin = (uint32_t)scalarStart[b];
-  }
-  out.gridletScalars.assign(scalarStart[numBricks], 0.f);
-  std::atomic<int64_t> conflicts{0};
-  parallelChunks(numBricks, [&](int64_t b) {
-    const Gridlet &g = out.gridlets[b];
-    const int nx = g.dims[0], ny = g.dims[1];
-    const int64_t ns = scalarStart[b + 1] - scalarStart[b];
-    std::vector<int32_t> ids(ns, -1);
-    static const int vtkOrder[8] = {0, 1, 3, 2, 4, 5, 7, 6};
-    int64_t bad = 0;
-    for (int64_t i = brickStart[b]; i < brickStart[b + 1]; ++i) {
-      const CubeRec &c = cubes[corder[i]];
-      const int bx = c.cid[0] - g.lower[0], by = c.cid[1] - g.lower[1], bz = c.cid[2] - g.lower[2];
-      for (int iz = 0; iz < 2; iz++)
-        for (int iy = 0; iy < 2; iy++)
-          for (int ix = 0; ix < 2; ix++) {
-            const int64_t idx = (bx + ix) + (int64_t)(nx + 1) * ((by + iy) + (int64_t)(ny + 1) * (bz + iz));
-            const int32_t sid = c.scalarIDs[vtkOrder[4 * iz + 2 * iy + ix]];
-            if (ids[idx] != -1 && ids[idx] != sid) bad++;
-            ids[idx] = sid;
-          }
-    }
-    float *dst = &out.gridletScalars[scalarStart[b]];
-    for (int64_t i = 0; i < ns; ++i)
-      dst[i] = ((uint32_t)ids[i] < (uint64_t)n) ? scalars[ids[i]] : NAN;
-    conflicts += bad;
-  });
-  out.conflicts += conflicts.load();
+  out.gridletScalarIDs.clear();
+  BrickSink sink;
+  sink.allocate = [&](int64_t total) {
+    out.gridletScalars.assign(total, 0.f);
+    if (keepIds) out.gridletScalarIDs.assign(total, -1);
+  };
+  sink.brick = [&](int64_t, int64_t begin, const int32_t *ids, int64_t ns) {
+    float *dst = &out.gridletScalars[begin];
+    for (int64_t i = 0; i < ns; ++i) dst[i] = ((uint32_t)ids[i] < (uint64_t)n) ? scalars[ids[i]] : NAN;
+    if (keepIds) memcpy(&out.gridletScalarIDs[begin], ids, ns * sizeof(int32_t));
+  };
+  int64_t conflicts = 0;
+  makeBricks(cubes, out.gridlets, sink, conflicts);
+  out.conflicts += conflicts;
+  if (keepIds) out.cubes.swap(cubes);
 }
 
 }  // namespace exa

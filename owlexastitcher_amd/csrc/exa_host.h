@@ -4,6 +4,7 @@
 // ExaStitchModel, and the point-location BVH.
 #pragma once
 #include <stdint.h>
+#include <functional>
 #include <string>
 #include <vector>
 #include "exa_types.h"
@@ -48,18 +49,43 @@ void amrShockCount(int baseRes, int baseLevel, float radius, float thickness, fl
 // ---------------------------------------------------------------------------
 // dual mesh (tools/amrMakeDualMesh/makeDual.cpp, makeGrids.cpp)
 // ---------------------------------------------------------------------------
+// A perfect dual cube: 8 same-level cells; cid = lowest cell position /
+// 2^level; scalarIDs in the emitter's corner order (makeDual.cpp:445-449,
+// 473-480).
+struct CubeRec {
+  int32_t level;
+  int32_t cid[3];
+  int32_t scalarIDs[8];
+};
+
 struct DualMesh {
   std::vector<float> vertices;     // float4 per vertex (cell centre, scalar)
   std::vector<int32_t> indices;    // int[8] per element, -1 padded
   std::vector<Gridlet> gridlets;
   std::vector<float> gridletScalars;
+  // kept on request (the on-disk outputs need them): per vertex its cell's
+  // scalar ID, per gridlet scalar the scalar ID or -1, the perfect cubes
+  std::vector<int32_t> vertexTags;
+  std::vector<int32_t> gridletScalarIDs;
+  std::vector<CubeRec> cubes;
   int64_t counts[4] = {0, 0, 0, 0};  // tets, pyramids, wedges, hexes
   int64_t numCubes = 0;
   int64_t conflicts = 0;   // brick scalar write conflicts + impossible cases
   int64_t degenerate = 0;  // dual cells dropped as degenerate
 };
 // Throws std::runtime_error on invalid input.
-void buildDualMesh(const AmrCell *cells, const float *scalars, int64_t n, DualMesh &out);
+void buildDualMesh(const AmrCell *cells, const float *scalars, int64_t n, DualMesh &out, bool keepIds = false);
+
+// Gridlets from perfect cubes (makeGrids.cpp:46-209).  allocate(total) is
+// called once with the number of gridlet scalars, then brick(b, begin, ids,
+// n) once per brick, from several threads, with the scalar IDs (-1: hole)
+// of brick b, whose scalars start at begin.
+struct BrickSink {
+  std::function<void(int64_t)> allocate;
+  std::function<void(int64_t, int64_t, const int32_t *, int64_t)> brick;
+};
+void makeBricks(const std::vector<CubeRec> &cubes, std::vector<Gridlet> &gridlets, const BrickSink &sink,
+                int64_t &conflicts);
 
 // ---------------------------------------------------------------------------
 // model (model/ExaStitchModel.cpp:24-257 in-memory form)

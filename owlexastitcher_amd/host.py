@@ -9,6 +9,7 @@ Device buffers are torch tensors on the current HIP device (plumbing only);
 every computation runs in libexastitch.so.
 """
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -25,6 +26,12 @@ RENDER_PERSISTENT = 1
 def _ptr(a):
     # data_as keeps a reference to the array for the lifetime of the pointer
     return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+def _path(p):
+    if p is None:
+        return None
+    return os.fsencode(p)
 
 
 def _f32(x, n=None):
@@ -80,6 +87,26 @@ class AmrScene:
         h = C.c_void_p()
         check(lib().exa_amr_create_from_cells(_ptr(cells), _ptr(scalars), len(cells), C.byref(h)))
         return cls(h)
+
+    @classmethod
+    def load(cls, cells_path, scalars_path):
+        """Leaf cells from a .cells file (16-byte {x, y, z, level} records) and
+        one float32 scalar per cell from a .scalars file."""
+        h = C.c_void_p()
+        check(lib().exa_amr_load(_path(cells_path), _path(scalars_path), C.byref(h)))
+        return cls(h)
+
+    def save(self, cells_path=None, scalars_path=None):
+        check(lib().exa_amr_save(self.h, _path(cells_path), _path(scalars_path)))
+
+    def write_dual_mesh(self, element_mesh_path, cubes_prefix=None, grids_path=None):
+        """amrMakeDualMesh + amrMakeGrids to files: the element mesh, the
+        per-level <cubes_prefix>_<L>.cubes files and the .grids file (None
+        skips a file).  Returns the dual mesh counts."""
+        c = np.zeros(11, np.int64)
+        check(lib().exa_amr_write_dual_mesh(self.h, _path(element_mesh_path), _path(cubes_prefix),
+                                            _path(grids_path), _ptr(c)))
+        return dict(zip(DualMesh.COUNT_NAMES, (int(v) for v in c)))
 
     def __len__(self):
         return int(lib().exa_amr_num_cells(self.h))
@@ -147,6 +174,16 @@ class ExaStitchModel:
         return cls(h)
 
     @classmethod
+    def load(cls, umesh_file_name, grids_file_name, scalar_file_name):
+        """ExaStitchModel::load(umesh, grids, scalars) (ExaStitchModel.cpp:24-257);
+        the element mesh is this library's EXAELEM1 file; any name may be
+        None or empty."""
+        h = C.c_void_p()
+        check(lib().exa_model_load(_path(umesh_file_name), _path(grids_file_name), _path(scalar_file_name),
+                                   C.byref(h)))
+        return cls(h)
+
+    @classmethod
     def from_arrays(cls, vertices, indices, gridlets, gridlet_scalars):
         v = _f32(vertices).reshape(-1)
         i = np.ascontiguousarray(indices, dtype=np.int32).reshape(-1)
@@ -168,6 +205,17 @@ class ExaStitchModel:
         c = np.zeros(4, np.int64)
         check(lib().exa_model_get_counts(self.h, _ptr(c)))
         return dict(zip(("vertices", "elements", "gridlets", "gridlet_scalars"), (int(x) for x in c)))
+
+    def arrays(self):
+        """(vertices float4 [V,4], indices int[8] [E,8], gridlets int32 [G,8],
+        gridlet scalars [S]) as held by the model."""
+        c = self.counts()
+        v = np.empty((c["vertices"], 4), np.float32)
+        i = np.empty((c["elements"], 8), np.int32)
+        g = np.empty((c["gridlets"], 8), np.int32)
+        s = np.empty(c["gridlet_scalars"], np.float32)
+        check(lib().exa_model_get(self.h, _ptr(v), _ptr(i), _ptr(g), _ptr(s)))
+        return v, i, g, s
 
     def setNumGridCells(self, num_mcs):
         d = np.ascontiguousarray(num_mcs, dtype=np.int32)
@@ -432,14 +480,31 @@ def tf_rasterize(points, N=128):
     return out
 
 
+def make_grids(cubes_paths, grids_path):
+    """amrMakeGrids (makeGrids.cpp:262-343): bricks of each _<L>.cubes file,
+    appended to one .grids file in argument order.  Returns the number of
+    inconsistent brick-vertex writes (0 for a valid dual mesh)."""
+    arr = (C.c_char_p * max(len(cubes_paths), 1))(*[_path(p) for p in cubes_paths])
+    bad = C.c_int64()
+    check(lib().exa_make_grids(arr, len(cubes_paths), _path(grids_path), C.byref(bad)))
+    return int(bad.value)
+
+
+def xf_save(path, xf):
+    """Writes a .xf file (the layout xf_load reads)."""
+    cm = _f32(xf["colorMap"]).reshape(-1, 4)
+    params = _f32([xf["opacityScale"], *xf["absDomain"], *xf["relDomain"]], 5)
+    check(lib().exa_xf_write(_path(path), _ptr(params), _ptr(cm), len(cm)))
+
+
 def xf_load(path):
     os_ = C.c_float()
     ad = np.zeros(2, np.float32)
     rd = np.zeros(2, np.float32)
     n = C.c_int()
-    check(lib().exa_xf_load(path.encode(), C.byref(os_), _ptr(ad), _ptr(rd), C.byref(n), None, 0))
+    check(lib().exa_xf_load(_path(path), C.byref(os_), _ptr(ad), _ptr(rd), C.byref(n), None, 0))
     cm = np.empty((n.value, 4), np.float32)
-    check(lib().exa_xf_load(path.encode(), None, None, None, None, _ptr(cm), n.value))
+    check(lib().exa_xf_load(_path(path), None, None, None, None, _ptr(cm), n.value))
     return {"opacityScale": os_.value, "absDomain": ad, "relDomain": rd, "colorMap": cm}
 
 

@@ -146,11 +146,8 @@ def test_tf_rasterize_hits_control_points():
 
 
 def test_xf_loader_on_reference_data():
-    """headless.cpp:258-283 .xf layout, on the reference's own TF files."""
-    path = "/root/reference/cloud.xf"
-    if not os.path.exists(path):
-        pytest.skip("reference data not mounted")
-    xf = xf_load(path)
+    """headless.cpp:258-283 .xf layout, on the reference's own TF file."""
+    xf = xf_load(os.path.join(ROOT, "tests", "golden", "cloud.xf"))
     assert xf["colorMap"].shape == (128, 4)
     assert xf["opacityScale"] == 100.0
     assert np.allclose(xf["absDomain"], [0, 1])
