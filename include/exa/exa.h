@@ -209,13 +209,27 @@ int exa_renderer_render(exa_renderer *r, uint32_t *d_fb, void *stream);
    TF fetches, neural-field lookups (synchronises the stream). */
 int exa_renderer_render_counted(exa_renderer *r, uint32_t *d_fb, uint64_t counters[11], void *stream);
 int exa_renderer_get_accum(const exa_renderer *r, float *hostOut); /* float4 per pixel */
-int exa_renderer_set_type(exa_renderer *r, int type); /* 0 path tracer, 1 direct lighting */
+/* OWLRenderer::Type (OWLRenderer.h:32): 0 path tracer, 1 direct lighting,
+   2 ray marcher (deviceCode.cu:1213-1260; runs in the one-thread-per-pixel
+   kernel whatever the render scheme) */
+int exa_renderer_set_type(exa_renderer *r, int type);
 int exa_renderer_set_color_map(exa_renderer *r, const float *rgba, int numColors);
 int exa_renderer_set_range(exa_renderer *r, float lo, float hi);
 int exa_renderer_set_rel_domain(exa_renderer *r, float lo, float hi);
 int exa_renderer_set_opacity_scale(exa_renderer *r, float scale); /* stores 1.1^(scale-100) */
 int exa_renderer_set_clip_plane(exa_renderer *r, int id, int enabled, const float N[3], float d);
+/* OWLRenderer::setShadeMode / shadeModes (OWLRenderer.cpp:769-774):
+   0 default, 1 gridlets (checkerboard over gridlet cells, elements red),
+   2 teaser (deviceCode.cu:625-697) */
 int exa_renderer_set_shade_mode(exa_renderer *r, int mode);
+/* ray-marcher step in voxel units (LaunchParams render.dt; the reference
+   fixes it at 1, OWLRenderer.cpp:616) */
+int exa_renderer_set_dt(exa_renderer *r, float dt);
+/* OWLRenderer heatMapEnabled / heatMapScale (OWLRenderer.h:160-161,
+   deviceCode.cu:1323-1327): each pixel blended with the colour of the clock
+   cycles it took times scale / spp; frames with it run in the
+   one-thread-per-pixel kernel */
+int exa_renderer_set_heat_map(exa_renderer *r, int enabled, float scale);
 int exa_renderer_set_light_source(exa_renderer *r, int id, const float pos[3], float intensity, int on);
 /* build-side addition: a light at infinity in direction dir (towards the light) */
 int exa_renderer_set_directional_light(exa_renderer *r, int id, const float dir[3], float intensity, int on);
@@ -275,6 +289,9 @@ typedef struct {
   float lightIntensity[4];
   int32_t lightOn[4];
   int32_t lightType[4];
+  float dt; /* ray-marcher step */
+  int32_t heatMapEnabled;
+  float heatMapScale;
 } exa_render_state;
 int exa_renderer_get_state(const exa_renderer *r, exa_render_state *out);
 void exa_renderer_destroy(exa_renderer *r);

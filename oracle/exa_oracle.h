@@ -140,7 +140,7 @@ typedef struct {
 
 typedef struct {
   int32_t W, H, spp, accumID;
-  int32_t integrator;      /* 0 path tracer, 1 direct lighting */
+  int32_t integrator;      /* 0 path tracer, 1 direct lighting, 2 ray marcher */
   int32_t maxBounces;      /* deviceCode.cu:1050 hard-codes 1024 */
   float   camOrg[3], camDir00[3], camDu[3], camDv[3];
   float   worldBounds[6];
@@ -159,6 +159,8 @@ typedef struct {
   float   clipD;
   ora_light lights[ORA_LIGHTS_MAX];
   int32_t debugCrosshair;  /* deviceCode.cu:1334-1342 (DEBUGGING == 1) */
+  int32_t shadeMode;       /* deviceCode.cu:619-623: 0 default, 1 gridlets, 2 teaser */
+  float   dt;              /* ray-marcher step (LaunchParams render.dt) */
 } ora_render_params;
 
 /* Render rows [y0,y1) of one frame.  accum is float4[W*H] (in/out),

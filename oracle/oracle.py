@@ -43,7 +43,7 @@ class OraRenderParams(C.Structure):
         ("tfDomain", C.c_float * 2), ("opacityScale", C.c_float),
         ("clipEnabled", C.c_int32), ("clipN", C.c_float * 3), ("clipD", C.c_float),
         ("lights", OraLight * 4),
-        ("debugCrosshair", C.c_int32),
+        ("debugCrosshair", C.c_int32), ("shadeMode", C.c_int32), ("dt", C.c_float),
     ]
 
 
@@ -234,7 +234,7 @@ def render_params_from_state(st, majorants, tf_rgba):
     """Fill oracle render params from an exa_render_state snapshot."""
     p = OraRenderParams()
     for f in ("W", "H", "spp", "accumID", "integrator", "maxBounces", "opacityScale", "clipEnabled", "clipD",
-              "tfN", "debugCrosshair"):
+              "tfN", "debugCrosshair", "shadeMode", "dt"):
         setattr(p, f, getattr(st, f))
     for f in ("camOrg", "camDir00", "camDu", "camDv", "worldBounds", "voxelXfm", "voxelXfmInv", "lightXfm",
               "gridDims", "gridBounds", "tfDomain", "clipN"):

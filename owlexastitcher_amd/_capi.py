@@ -69,6 +69,7 @@ class RenderState(C.Structure):
         ("clipEnabled", C.c_int32), ("clipN", C.c_float * 3), ("clipD", C.c_float),
         ("lightPos", (C.c_float * 3) * 4), ("lightIntensity", C.c_float * 4),
         ("lightOn", C.c_int32 * 4), ("lightType", C.c_int32 * 4),
+        ("dt", C.c_float), ("heatMapEnabled", C.c_int32), ("heatMapScale", C.c_float),
     ]
 
 
@@ -136,6 +137,8 @@ _SIGS = {
     "exa_renderer_set_opacity_scale": (C.c_int, [P, F]),
     "exa_renderer_set_clip_plane": (C.c_int, [P, C.c_int, C.c_int, P, F]),
     "exa_renderer_set_shade_mode": (C.c_int, [P, C.c_int]),
+    "exa_renderer_set_dt": (C.c_int, [P, F]),
+    "exa_renderer_set_heat_map": (C.c_int, [P, C.c_int, F]),
     "exa_renderer_set_light_source": (C.c_int, [P, C.c_int, P, F, C.c_int]),
     "exa_renderer_set_directional_light": (C.c_int, [P, C.c_int, P, F, C.c_int]),
     "exa_renderer_set_light_space_transform": (C.c_int, [P, P]),

@@ -925,6 +925,9 @@ int exa_renderer_create(const exa_model *m, const int32_t numMCs[3], const float
     P.shadeMode = 0;
     P.maxBounces = 1024;
     P.debugCrosshair = 1;
+    P.dt = 1.f;  // OWLRenderer.cpp:616 sets render.dt = 1 for every frame
+    P.heatMapEnabled = 0;
+    P.heatMapScale = 1.f;
     P.opacityScale = 1.f;
     P.spp = 1;
     *out = r.release();
@@ -960,7 +963,10 @@ static void doRender(exa_renderer *r, uint32_t *fb, unsigned long long *counters
   P.spp = r->spp < 1 ? 1 : r->spp;
   P.tf = r->tf.ptr;
   P.tfN = (int32_t)r->tf.count;
-  if (r->scheme == EXA_RENDER_PERSISTENT) {
+  // the ray marcher and the heat map (a per-pixel clock) run in the
+  // one-thread-per-pixel kernel
+  const bool mega = r->scheme != EXA_RENDER_PERSISTENT || P.integrator == 2 || P.heatMapEnabled;
+  if (!mega) {
     r->work.alloc(1);
     const bool useNet = r->neural && r->neural->validated;
     if (useNet && r->sampler.accel != kAccelTables)
@@ -970,7 +976,8 @@ static void doRender(exa_renderer *r, uint32_t *fb, unsigned long long *counters
     launchRenderPersistent(P, r->sampler.view(), r->sampler.accel, r->sampler.gridView(), r->work.ptr, r->tuning,
                            counters, s, useNet ? &nv : nullptr, useNet ? r->neural->flags.ptr : nullptr);
   } else {
-    if (r->neural && r->neural->validated) throw Status(EXA_ERR_STATE, "the neural field needs the persistent scheme");
+    if (r->neural && r->neural->validated && P.integrator != 2)
+      throw Status(EXA_ERR_STATE, "the neural field needs the persistent scheme");
     launchRender(P, r->sampler.view(), r->sampler.accel, r->sampler.gridView(), counters, s);
   }
   HIP_CHECK(hipGetLastError());
@@ -1004,7 +1011,7 @@ int exa_renderer_get_accum(const exa_renderer *r, float *out) {
 int exa_renderer_set_type(exa_renderer *r, int type) {
   return guarded([&] {
     require(r, "renderer is NULL");
-    require(type == 0 || type == 1, "supported integrators: 0 path tracer, 1 direct lighting");
+    require(type >= 0 && type <= 2, "integrators: 0 path tracer, 1 direct lighting, 2 ray marcher");
     r->P.integrator = type;
     r->accumID = 0;
   });
@@ -1054,8 +1061,24 @@ int exa_renderer_set_clip_plane(exa_renderer *r, int id, int enabled, const floa
 int exa_renderer_set_shade_mode(exa_renderer *r, int mode) {
   return guarded([&] {
     require(r, "renderer is NULL");
-    require(mode == 0, "only the default shade mode is implemented");
+    require(mode >= 0 && mode <= 2, "shade modes: 0 default, 1 gridlets, 2 teaser");
     r->P.shadeMode = mode;
+    r->accumID = 0;
+  });
+}
+int exa_renderer_set_dt(exa_renderer *r, float dt) {
+  return guarded([&] {
+    require(r, "renderer is NULL");
+    require(dt > 0.f && dt < 1e30f, "dt must be positive and finite");
+    r->P.dt = dt;
+    r->accumID = 0;
+  });
+}
+int exa_renderer_set_heat_map(exa_renderer *r, int enabled, float scale) {
+  return guarded([&] {
+    require(r, "renderer is NULL");
+    r->P.heatMapEnabled = enabled ? 1 : 0;
+    r->P.heatMapScale = scale;
     r->accumID = 0;
   });
 }
@@ -1745,6 +1768,9 @@ int exa_renderer_get_state(const exa_renderer *r, exa_render_state *o) {
       o->lightOn[i] = P.lights[i].on;
       o->lightType[i] = P.lights[i].type;
     }
+    o->dt = P.dt;
+    o->heatMapEnabled = P.heatMapEnabled;
+    o->heatMapScale = P.heatMapScale;
   });
 }
 void exa_renderer_destroy(exa_renderer *r) { delete r; }

@@ -96,10 +96,13 @@ struct LightParams {
 // LaunchParams (LaunchParams.h:56-119) restricted to the ExaStitch path.
 struct RenderParams {
   int32_t W, H, spp, accumID;
-  int32_t integrator;  // 0 path tracer, 1 direct lighting
-  int32_t shadeMode;   // 0 default
+  int32_t integrator;  // 0 path tracer, 1 direct lighting, 2 ray marcher
+  int32_t shadeMode;   // 0 default, 1 gridlet checkerboard, 2 teaser
   int32_t maxBounces;
   int32_t debugCrosshair;
+  float dt;                // ray-marcher step (voxel units)
+  int32_t heatMapEnabled;  // per-pixel clock cycles shown over the image
+  float heatMapScale;
   float camOrg[3], camDir00[3], camDu[3], camDv[3];
   float worldBounds[6];
   float voxelXfm[12];

@@ -196,14 +196,50 @@ EXA_DEV bool trackingStep(float &t, float majorant, float os, float t1, Lcg &rng
   return !(t >= t1);
 }
 
+// deviceCode.cu:625-697 classifySample, shade modes 1 (Gridlets) and 2
+// (Teaser): the colour of a gridlet sample becomes a checkerboard over the
+// gridlet's cells (white / dark blue), an element sample turns red; Teaser
+// blends the TF colour into that colour across a diagonal band of the image.
+// Alpha stays the TF's, so tracking and the random stream are unchanged.
+EXA_DEV float4 shadeSample(const RenderParams &P, const Gridlet *gridlets, const Sample &s, float4 xf, int px,
+                           int py) {
+  vec3f rgb2;
+  if (s.cellID == -1) {
+    rgb2 = mk3(1.f, 0.f, 0.f);
+  } else {
+    const Gridlet &g = gridlets[s.primID];
+    const int ix = s.cellID % g.dims[0];
+    const int iy = s.cellID / g.dims[0] % g.dims[1];
+    const int iz = s.cellID / (g.dims[0] * g.dims[1]);
+    int col = (ix % 2 == iy % 2) ? 1 : 0;
+    col = iz % 2 == 0 ? col : !col;
+    rgb2 = col == 0 ? mk3(1.f, 1.f, 1.f) : mk3(0.f, 0.f, 0.7f);
+  }
+  if (P.shadeMode == 1) return make_float4(rgb2.x, rgb2.y, rgb2.z, xf.w);
+  const float u = (float)px / (float)P.W, v = (float)py / (float)P.H;
+  const float ax = 0.4f, ay = 0.6f, bx = 0.6f, by = 0.4f;
+  const float A = bx - ax, B = by - ay;
+  const float C1 = A * ax + B * ay;
+  const float C2 = A * bx + B * by;
+  const float C = A * u + B * v;
+  if (C <= C1) return xf;
+  if (C >= C2) return make_float4(rgb2.x, rgb2.y, rgb2.z, xf.w);
+  return make_float4((xf.x * (C2 - C) + rgb2.x * (C - C1)) / (C2 - C1),
+                     (xf.y * (C2 - C) + rgb2.y * (C - C1)) / (C2 - C1),
+                     (xf.z * (C2 - C) + rgb2.z * (C - C1)) / (C2 - C1), xf.w);
+}
+
 // A located sample at a tentative collision (deviceCode.cu:742-789 for
 // Woodcock, 800-833 for ratio tracking).  Returns true when the segment
 // ends in an interaction; primID caches the last gridlet for testSample.
-EXA_DEV bool resolveCollision(const RenderParams &P, const Sample &s, bool shadow, float majorant, Lcg &rng,
-                              int32_t &primID, float4 &xf, float &Tr) {
+// (px, py) is the pixel, for the Teaser shade mode.
+EXA_DEV bool resolveCollision(const RenderParams &P, const Gridlet *gridlets, int px, int py, const Sample &s,
+                              bool shadow, float majorant, Lcg &rng, int32_t &primID, float4 &xf, float &Tr) {
   if (s.primID < 0) return false;
   if (s.cellID != -1) primID = s.primID;
   xf = lookupTF(P, s.value);
+  // ratio tracking reads alpha only, so only Woodcock needs the shade mode
+  if (P.shadeMode != 0 && !shadow) xf = shadeSample(P, gridlets, s, xf, px, py);
   const float os = P.opacityScale;
   if (!shadow) {
     const float u = rng();
@@ -218,6 +254,35 @@ EXA_DEV bool resolveCollision(const RenderParams &P, const Sample &s, bool shado
     if (Tr <= 0.f) return true;
   }
   return false;
+}
+
+// deviceCode.cu:151-154 over
+EXA_DEV float4 over(float4 a, float4 b) {
+  const float k = 1.f - a.w;
+  return make_float4(a.x + k * b.x, a.y + k * b.y, a.z + k * b.z, a.w + k * b.w);
+}
+
+// deviceCode.cu:104-112 firstSampleT: the first sample at or after t0 on the
+// lattice dt * (ils + k)
+EXA_DEV float firstSampleT(float t0, float dt, float ils) {
+  const float numSegs = floorf((t0 - dt * ils) / dt);
+  float t = dt * (ils + numSegs);
+  if (t < t0) t += dt;
+  return t;
+}
+
+// deviceCode.cu:155-185 heatMap = temperature_to_rgb (hue from blue to red,
+// value from .5 to 1)
+EXA_DEV vec3f heatMapColor(float t) {
+  auto sat = [](float x) { return fminf(fmaxf(x, 0.f), 1.f); };
+  const float K = 4.0f / 6.0f;
+  const float h = K - K * t;
+  const float v = .5f + 0.5f * t;
+  const float hs = sat(h) * 6.0f;
+  const float r = sat(fabsf(hs - 3.f) - 1.0f);
+  const float g = sat(2.0f - fabsf(hs - 2.0f));
+  const float b = sat(2.0f - fabsf(hs - 4.0f));
+  return mk3(v * r, v * g, v * b);
 }
 
 // deviceCode.cu:856-926 sampleLight up to the shadow ray (+ directional

@@ -19,6 +19,7 @@ struct Ctx {
   Lcg &rng;
   int32_t *stack;
   LocalCounters &cnt;
+  int px, py;
 };
 
 struct Interaction {
@@ -62,7 +63,7 @@ EXA_DEV Interaction sampleInteraction(Ctx<kCount, kAccel> &C, const Ray &ray, bo
           if (kCount) C.cnt.c[kind < 0 ? kCntMiss : kCntGridletHit + kind]++;
         }
         if (kCount && s.primID >= 0) C.cnt.c[kCntTfFetch]++;
-        if (resolveCollision(P, s, shadow, majorant, C.rng, primID, I.xf, I.Tr)) {
+        if (resolveCollision(P, C.sv.gridlets, C.px, C.py, s, shadow, majorant, C.rng, primID, I.xf, I.Tr)) {
           I.type = kScattering;
           goOn = false;
           break;
@@ -130,6 +131,54 @@ EXA_DEV float4 directLight(Ctx<kCount, kAccel> &C, Ray ray, float4 bg, int numLi
   return make_float4(throughput.x, throughput.y, throughput.z, 1.f);
 }
 
+// Ray marcher (deviceCode.cu:1213-1260 rayMarchingIntegrate, 348-398
+// integrateDVR with Shading): absorption + emission at samples dt apart on a
+// lattice jittered by one random number, front-to-back compositing of the
+// raw TF colour and alpha (no opacity scale), stopped at alpha 0.99, over the
+// background.  The reference reads an uninitialised gradient for lit
+// samples; here the gradient is zero, so a lit sample keeps the 0.1 ambient
+// term.  At most kMaxMarchSteps samples per ray.
+constexpr int kMaxMarchSteps = 1 << 20;
+
+template <bool kCount, int kAccel>
+EXA_DEV float4 rayMarch(Ctx<kCount, kAccel> &C, Ray ray, float4 bg, int numLights) {
+  const RenderParams &P = C.P;
+  float4 color = bg;
+  float t0 = 1e30f, t1 = -1e30f;
+  if (!boxTest(ray, P.worldBounds, t0, t1)) return color;
+  clipInterval(P, ray, t0, t1);
+  const float ils = C.rng();
+  // integrateDVR: the ray goes to voxel space; t0 and t1 stay as they are
+  const vec3f org = xfmPoint(P.voxelXfm, ray.org);
+  const vec3f dir = normalize(xfmVector(P.voxelXfm, ray.dir));
+  const float dt = P.dt;
+  float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+  int steps = 0;
+  for (float t = firstSampleT(t0, dt, ils); t <= t1 && steps < kMaxMarchSteps; t += dt, ++steps) {
+    const vec3f pos = org + dir * t;
+    int kind;
+    const Sample s = locatePoint<kAccel>(C.sv, pos, C.stack, kRenderBlock, kind);
+    if (kCount) C.cnt.c[kind < 0 ? kCntMiss : kCntGridletHit + kind]++;
+    if (s.primID < 0) continue;
+    if (kCount) C.cnt.c[kCntTfFetch]++;
+    const float4 xf = lookupTF(P, s.value);
+    const vec3f cd = mk3(xf.x, xf.y, xf.z);
+    vec3f shaded = cd * .1f;
+    if (numLights == 0) shaded = cd;
+    for (int i = 0; i < numLights; ++i) {
+      const vec3f lp = ld3(P.lights[i].pos);
+      const vec3f L = normalize(lp - pos);
+      const float ld = length(lp - pos);
+      const float NdotL = 0.f * L.x + 0.f * L.y + 0.f * L.z;
+      shaded = shaded + cd * (.9f * NdotL / (ld * ld) * P.lights[i].intensity);
+    }
+    const float k = (1.f - acc.w) * xf.w;
+    acc = make_float4(acc.x + k * shaded.x, acc.y + k * shaded.y, acc.z + k * shaded.z, acc.w + k * 1.f);
+    if (acc.w >= 0.99f) break;
+  }
+  return over(acc, color);
+}
+
 template <bool kCount, int kAccel>
 __global__ __launch_bounds__(kRenderBlock) void kRender(RenderParams P, SamplerView sv, GridView gv,
                                                         unsigned long long *counters) {
@@ -144,17 +193,26 @@ __global__ __launch_bounds__(kRenderBlock) void kRender(RenderParams P, SamplerV
   Lcg rng;
   const int pixelID = px + P.W * py;
   rng.init((uint32_t)pixelID, (uint32_t)P.accumID);
-  Ctx<kCount, kAccel> C{P, sv, gv, rng, stack + tid, cnt};
+  Ctx<kCount, kAccel> C{P, sv, gv, rng, stack + tid, cnt, px, py};
   const float4 bg = backgroundColor(py, P.H);
   const int numLights = countLights(P);
+  const uint64_t clockBegin = P.heatMapEnabled ? clock64() : 0;
   float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
   for (int s = 0; s < P.spp; ++s) {
     const Ray ray = primaryRay(P, px, py, rng);
-    const float4 c = P.integrator == 1 ? directLight(C, ray, bg, numLights) : pathTrace(C, ray, bg, numLights);
+    const float4 c = P.integrator == 1   ? directLight(C, ray, bg, numLights)
+                     : P.integrator == 2 ? rayMarch(C, ray, bg, numLights)
+                                         : pathTrace(C, ray, bg, numLights);
     acc.x += c.x;
     acc.y += c.y;
     acc.z += c.z;
     acc.w += c.w;
+  }
+  if (P.heatMapEnabled) {
+    // deviceCode.cu:1323-1327: cycles spent on this pixel, over the samples
+    const float t = (float)(clock64() - clockBegin) * (P.heatMapScale / (float)P.spp);
+    const vec3f h = heatMapColor(t);
+    acc = over(make_float4(h.x, h.y, h.z, .5f), acc);
   }
   finishPixel(P, px, py, acc);
   if (kCount) {

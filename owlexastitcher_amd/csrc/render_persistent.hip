@@ -394,7 +394,7 @@ kRenderPersistent(RenderParams P, SamplerView sv, GridView gv, PersistParams pp,
     if (have) {
       if (kCount && smp.primID >= 0) cnt.c[kCntTfFetch]++;
       const bool shadow = L.kind == kSegPtShadow || L.kind == kSegDlShadow;
-      if (resolveCollision(Pl, smp, shadow, L.majorant, L.rng, L.primID, L.xf, L.Tr)) {
+      if (resolveCollision(Pl, sv.gridlets, L.px, L.py, smp, shadow, L.majorant, L.rng, L.primID, L.xf, L.Tr)) {
         L.itype = kScattering;
         L.st = kStSegEnd;
       } else {

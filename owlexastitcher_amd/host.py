@@ -17,6 +17,7 @@ from ._capi import (AmrCell, Gridlet, RenderState, StitchModelDesc, check, lib)
 
 PATH_TRACER = 0
 DIRECT_LIGHTING = 1
+RAY_MARCHER = 2
 ACCEL_TABLES = 0
 ACCEL_BVH = 1
 RENDER_MEGAKERNEL = 0
@@ -381,6 +382,17 @@ class OWLRenderer:
 
     def setShadeMode(self, sm):
         check(lib().exa_renderer_set_shade_mode(self.h, int(sm)))
+
+    @staticmethod
+    def shadeModes():
+        return {0: "default", 1: "gridlets", 2: "teaser"}
+
+    def setDt(self, dt):
+        """Ray-marcher step (the reference fixes it at 1)."""
+        check(lib().exa_renderer_set_dt(self.h, float(dt)))
+
+    def setHeatMap(self, enabled, scale=1.0):
+        check(lib().exa_renderer_set_heat_map(self.h, int(bool(enabled)), float(scale)))
 
     def setLightSource(self, i, pos, intensity, on):
         check(lib().exa_renderer_set_light_source(self.h, int(i), _ptr(_f32(pos, 3)), float(intensity),

@@ -99,6 +99,72 @@ def test_render_matches_oracle_f32(cfg1, device, integrator, light, accel, schem
     assert not bgmask.all()
 
 
+@pytest.mark.parametrize("scheme", SCHEMES, ids=["persistent", "megakernel"])
+@pytest.mark.parametrize("shade", [1, 2], ids=["gridlets", "teaser"])
+def test_shade_modes_match_oracle(cfg1, device, shade, scheme):
+    """Shade modes (deviceCode.cu:625-697) recolour samples but keep alpha, so
+    the paths (and the alpha channel) are those of the default mode."""
+    print(f"case: shade mode {shade}, scheme {scheme}, 64x48", flush=True)
+    r = make_renderer(cfg1, 64, 48)
+    r.setScheme(scheme)
+    r.setMaxBounces(4)
+    r.setDirectionalLight(0, (0.3, 1.0, 0.5), 2.0, True)
+    acc_d, _, _, _ = render_both(cfg1, r, 1, device)
+    r.setShadeMode(shade)
+    r.accumID = 0
+    acc, acc_o, fb, fb_o = render_both(cfg1, r, 1, device)
+    assert np.array_equal(acc, acc_o), _report(acc, acc_o)
+    assert np.array_equal(fb, fb_o)
+    assert np.array_equal(acc[..., 3], acc_d[..., 3])
+    assert not np.array_equal(acc[..., :3], acc_d[..., :3])
+
+
+@pytest.mark.parametrize("accel", [ACCEL_TABLES, ACCEL_BVH], ids=["tables", "bvh"])
+@pytest.mark.parametrize("light,dt", [("none", 1.0), ("point", 0.37), ("none", 2.5)])
+def test_ray_marcher_matches_oracle(cfg1, device, accel, light, dt):
+    """Ray marcher (deviceCode.cu:1213-1260, 348-398), bit-exact, 2 frames."""
+    print(f"case: ray marcher accel={accel} light={light} dt={dt}, 64x48", flush=True)
+    r = make_renderer(cfg1, 64, 48)
+    r.setAccel(accel)
+    r.setType(2)
+    r.setDt(dt)
+    if light == "point":
+        b = r.modelBounds
+        r.setLightSource(0, b[3:] + 10.0, 60.0 * 500.0, True)
+    acc, acc_o, fb, fb_o = render_both(cfg1, r, 2, device)
+    assert np.array_equal(acc, acc_o), _report(acc, acc_o)
+    assert np.array_equal(fb, fb_o)
+    bgmask = np.all(acc == acc[:, :1], axis=(1, 2))
+    assert not bgmask.all()
+
+
+def test_heat_map(cfg1, device):
+    """Heat map (deviceCode.cu:1323-1327): over(heat colour with alpha .5,
+    sample sum), so alpha becomes .5 + .5 alpha and the colour stays within
+    [.5 c, .5 c + 1]."""
+    import torch
+    print("case: heat map, 64x48", flush=True)
+    r = make_renderer(cfg1, 64, 48)
+    r.setMaxBounces(4)
+    fb = torch.zeros((48, 64), dtype=torch.int32, device=device)
+    r.render(fb)
+    torch.cuda.synchronize()
+    a0 = r.accum()
+    r.setHeatMap(True, 1e-6)
+    assert r.accumID == 0
+    r.render(fb)
+    torch.cuda.synchronize()
+    a1 = r.accum()
+    assert np.all(np.isfinite(a1))
+    assert np.array_equal(a1[..., 3], np.float32(0.5) + np.float32(0.5) * a0[..., 3])
+    assert np.all(a1[..., :3] >= np.float32(0.5) * a0[..., :3]) and np.all(a1[..., :3] <= 0.5 * a0[..., :3] + 1.0)
+    assert not np.array_equal(a1, a0)
+    r.setHeatMap(False)
+    r.render(fb)
+    torch.cuda.synchronize()
+    assert np.array_equal(r.accum(), a0)
+
+
 @pytest.mark.parametrize("tuning", [(1, 0, 0), (1, 64, 64), (64, 32, 0), (3, 17, 64), (5, 0, 40)])
 def test_persistent_tuning_matches_oracle(cfg1, device, tuning):
     """Images do not depend on the persistent renderer's scheduling knobs;
