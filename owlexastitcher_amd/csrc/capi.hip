@@ -361,6 +361,7 @@ struct exa_neural {
   DeviceBuffer<float2> widened;
   float bound = 0.f;
   bool validated = false;
+  uint64_t generation = 0;  // bumped whenever flags / widened ranges / parameters change
 
   static constexpr int64_t kW1 = kNeuralIn * kNeuralHidden, kW2 = kNeuralHidden * kNeuralHidden,
                            kW3 = kNeuralHidden;
@@ -445,6 +446,9 @@ struct exa_batch {
   std::vector<std::unique_ptr<Req>> reqs;
   std::vector<BatchRequest> host;  // mirror of the device array (accumIDs advanced on both)
   DeviceBuffer<BatchRequest> dev;
+  // the neural field the TF majorants were widened for (nullptr: exact)
+  const exa_neural *majNeural = nullptr;
+  uint64_t majGeneration = 0;
   DeviceBuffer<uint32_t> work;
   uint32_t numWork = 0;
   bool dirty = true;
@@ -478,9 +482,33 @@ struct exa_batch {
     work.alloc(1);
     dirty = false;
   }
+  // With a validated neural field on the renderer, every TF's majorants come
+  // from the macrocell value ranges widened by the validation bound where the
+  // network answers (as the renderer's own); without one, from the exact
+  // ranges.  Recomputed when that choice or the field's validation changes.
+  const exa_neural *activeNeural() const {
+    return r->neural && r->neural->validated && r->sampler.accel == kAccelTables ? r->neural : nullptr;
+  }
+  void syncMajorants() {
+    const exa_neural *n = activeNeural();
+    if (n == majNeural && (!n || n->generation == majGeneration)) return;
+    const exa_sampler &S = r->sampler;
+    const int64_t nmc = (int64_t)S.dims[0] * S.dims[1] * S.dims[2];
+    for (auto &t : tfs)
+      launchMapMaxOpacity(n ? n->widened.ptr : S.valueRanges.ptr, nmc, t->colorMap.ptr, t->n, t->lo, t->hi,
+                          t->majorants.ptr, 0);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipStreamSynchronize(0));
+    majNeural = n;
+    majGeneration = n ? n->generation : 0;
+    dropGraph();  // a captured frame would keep the old network state
+  }
   void launch(unsigned long long *counters, hipStream_t s) {
+    const exa_neural *n = activeNeural();
+    NeuralView nv{};
+    if (n) nv = n->view();
     launchRenderBatch(r->sampler.view(), r->sampler.accel, r->sampler.gridView(), dev.ptr, (int)host.size(),
-                      numWork, work.ptr, tuning, counters, s);
+                      numWork, work.ptr, tuning, counters, s, n ? &nv : nullptr, n ? n->flags.ptr : nullptr);
   }
   void advanced() {
     for (BatchRequest &b : host) b.P.accumID += 1;
@@ -1230,6 +1258,8 @@ int exa_batch_add_transfer_function(exa_batch *b, const float *rgba, int numColo
     else S.haveMaxOp = hadMaxOp;
     *index = (int)b->tfs.size();
     b->tfs.push_back(std::move(t));
+    b->majNeural = nullptr;  // the new TF has exact majorants; widen all on the next frame if needed
+    b->majGeneration = 0;
   });
 }
 int exa_batch_add_request(exa_batch *b, const exa_batch_request *q, int *index) {
@@ -1305,6 +1335,7 @@ int exa_batch_render(exa_batch *b, void *stream) {
     require(b, "batch is NULL");
     if (b->r->tf.count == 0 && b->tfs.empty()) throw Status(EXA_ERR_STATE, "no transfer function");
     b->sync();
+    b->syncMajorants();
     b->launch(nullptr, (hipStream_t)stream);
     HIP_CHECK(hipGetLastError());
     b->advanced();
@@ -1314,6 +1345,7 @@ int exa_batch_render_counted(exa_batch *b, uint64_t counters[11], void *stream) 
   return guarded([&] {
     require(b && counters, "invalid arguments");
     b->sync();
+    b->syncMajorants();
     hipStream_t st = (hipStream_t)stream;
     unsigned long long *dc = b->r->sampler.counters.ptr;
     HIP_CHECK(hipMemsetAsync(dc, 0, kNumCounters * sizeof(unsigned long long), st));
@@ -1330,6 +1362,7 @@ int exa_batch_capture_graph(exa_batch *b, void *stream) {
   return guarded([&] {
     require(b && stream, "a non-default stream is needed for capture");
     b->sync();
+    b->syncMajorants();
     b->dropGraph();
     hipStream_t st = (hipStream_t)stream;
     HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
@@ -1349,6 +1382,9 @@ int exa_batch_render_graph(exa_batch *b, void *stream) {
   return guarded([&] {
     require(b, "batch is NULL");
     if (!b->exec || b->dirty) throw Status(EXA_ERR_STATE, "no captured graph (capture after the last change)");
+    const exa_neural *n = b->activeNeural();
+    if (n != b->majNeural || (n && n->generation != b->majGeneration))
+      throw Status(EXA_ERR_STATE, "the neural field changed since the capture (capture again)");
     HIP_CHECK(hipGraphLaunch(b->exec, (hipStream_t)stream));
     b->advanced();
   });
@@ -1479,6 +1515,7 @@ int exa_neural_create(exa_renderer *r, int levels, int log2T, float nMin, float 
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipDeviceSynchronize());
     n->validated = false;
+    n->generation++;
     *out = n.release();
   });
 }
@@ -1524,6 +1561,7 @@ int exa_neural_set_params(exa_neural *n, const float *table, const float *W1, co
     neuralZero(n->v.ptr, n->numParams, 0);
     n->step = 0;
     n->validated = false;
+    n->generation++;
     HIP_CHECK(hipDeviceSynchronize());
   });
 }
@@ -1599,6 +1637,7 @@ int exa_neural_train(exa_neural *n, int steps, int batch, float lr, float lrEnd,
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipDeviceSynchronize());
     n->validated = false;
+    n->generation++;
     if (lossOut && steps > 0) {
       HIP_CHECK(hipMemcpy(lossOut, n->lossBuf.ptr, steps * sizeof(float), hipMemcpyDeviceToHost));
       for (int i = 0; i < steps; ++i) lossOut[i] /= (float)batch;
@@ -1708,6 +1747,7 @@ int exa_neural_validate(exa_neural *n, exa_renderer *r, int pointsPerMC, float b
     }
     n->bound = bound;
     n->validated = true;
+    n->generation++;
     if (out) {
       out[0] = (float)used;
       out[1] = (float)withElems;

@@ -157,8 +157,10 @@ void launchRenderPersistent(const RenderParams &p, const SamplerView &sv, int ac
                             const NeuralView *nv = nullptr, const uint8_t *netFlags = nullptr);
 // Batched launch over reqs[0..numReqs) (device array); numWork = total work
 // ids.  Afterwards every request's accumID is advanced by one on the device.
+// With nv, the network answers the element stage of the macrocells flagged in
+// netFlags (tables accelerator).
 void launchRenderBatch(const SamplerView &sv, int accel, const GridView &gv, BatchRequest *reqs, int numReqs,
                        uint32_t numWork, uint32_t *work, const PersistTuning &t, unsigned long long *counters,
-                       hipStream_t s);
+                       hipStream_t s, const NeuralView *nv = nullptr, const uint8_t *netFlags = nullptr);
 
 }  // namespace exa

@@ -446,9 +446,9 @@ template <bool kBatch>
 static void launchAny(const RenderParams &p, const SamplerView &sv, int accel, const GridView &gv,
                       const PersistParams &pp, unsigned long long *counters, hipStream_t s) {
   hipLaunchKernelGGL(kResetWork, dim3(1), dim3(64), 0, s, pp.work);
-  if (!kBatch && pp.netFlags) {  // neural field: tables accelerator, single view
-    if (counters) launchPersistentT<true, kAccelTables, false, true>(p, sv, gv, pp, counters, s);
-    else launchPersistentT<false, kAccelTables, false, true>(p, sv, gv, pp, counters, s);
+  if (pp.netFlags) {  // neural field: tables accelerator
+    if (counters) launchPersistentT<true, kAccelTables, kBatch, true>(p, sv, gv, pp, counters, s);
+    else launchPersistentT<false, kAccelTables, kBatch, true>(p, sv, gv, pp, counters, s);
     return;
   }
   if (accel == kAccelBvh) {
@@ -485,10 +485,10 @@ void launchRenderPersistent(const RenderParams &p, const SamplerView &sv, int ac
 
 void launchRenderBatch(const SamplerView &sv, int accel, const GridView &gv, BatchRequest *reqs, int numReqs,
                        uint32_t numWork, uint32_t *work, const PersistTuning &t, unsigned long long *counters,
-                       hipStream_t s) {
+                       hipStream_t s, const NeuralView *nv, const uint8_t *netFlags) {
   PersistParams pp;
-  pp.nv = NeuralView{};
-  pp.netFlags = nullptr;
+  pp.nv = nv ? *nv : NeuralView{};
+  pp.netFlags = nv ? netFlags : nullptr;
   pp.work = work;
   pp.reqs = reqs;
   pp.numReqs = numReqs;

@@ -186,3 +186,48 @@ def test_validate_and_render_with_network(trained, renderer, device):
     assert c["neural"] > 0
     assert np.all(np.isfinite(N))
     assert err <= noise
+
+
+def test_batch_with_network_matches_single_view(trained, renderer, device):
+    """The batched launch runs the network in the element stage of the
+    flagged macrocells exactly as the single-view renderer does, with TF
+    majorants widened per transfer function; removing the network restores
+    the exact batch images."""
+    import torch
+    from owlexastitcher_amd import RenderBatch
+    f, _ = trained
+    vr = renderer.valueRange
+    f.validate(points_per_mc=64, bound=2e-2 * float(vr[1] - vr[0]))
+    W, H = renderer.size
+    st = renderer.state()
+    cam = (np.array(st.camOrg), np.array(st.camDir00), np.array(st.camDu), np.array(st.camDv))
+    b = RenderBatch(renderer)
+    t = b.addTransferFunction(scenes.tf128(), st.tfDomain[0], st.tfDomain[1], st.opacityScale)
+    b.addRequest(W, H, 1, t, cam, 0)
+    b.addRequest(33, 21, 2, t, cam, 0)
+
+    def single(neural):
+        renderer.setNeural(f if neural else None)
+        renderer.setSpp(1)
+        renderer.accumID = 0
+        fb = torch.zeros((H, W), dtype=torch.int32, device=device)
+        for _ in range(2):
+            renderer.render(fb)
+        torch.cuda.synchronize()
+        return renderer.accum()
+
+    want_net = single(True)  # leaves the network on the renderer
+    b.resetAccum()
+    c = b.render_counted()
+    b.render()
+    torch.cuda.synchronize()
+    print(f"batch counters with the network: {c}", flush=True)
+    assert c["neural"] > 0
+    assert np.array_equal(b.accum(0), want_net)
+    want_exact = single(False)
+    b.resetAccum()
+    c = b.render_counted()
+    b.render()
+    torch.cuda.synchronize()
+    assert c["neural"] == 0
+    assert np.array_equal(b.accum(0), want_exact)
