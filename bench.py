@@ -327,6 +327,142 @@ def neural_workload(r, args, dev):
     }
 
 
+TF_SHOCK = np.array([
+    [0.00, 0.1, 0.1, 0.3, 0.00],
+    [0.12, 0.2, 0.3, 0.9, 0.02],
+    [0.35, 0.1, 0.8, 0.9, 0.10],
+    [0.60, 0.9, 0.6, 0.2, 0.40],
+    [1.00, 1.0, 0.9, 0.8, 0.90],
+], np.float32)
+
+
+def large_workload(args, dev):
+    """The large configuration: 5-level AMR, base 256^3 cells of width 16 over
+    [0,4096]^3, refined along an expanding spherical shock shell (radius 1200,
+    thickness 40) with turbulence in its wake (about 400M leaf cells); the
+    neural field retrained on it; batched frames of one 3840x2160 view plus 16
+    side views at 1024^2, 64 spp each, in one persistent launch, with the exact
+    element stage and with the network.  --large-scale k divides the base
+    resolution and the radius by k (a smaller copy of the same scene)."""
+    import torch
+    from owlexastitcher_amd import (AmrScene, DualMesh, ExaStitchModel, NeuralField, OWLRenderer, RenderBatch,
+                                    camera_look_at, tf_rasterize)
+    k = args.large_scale
+    t = time.time()
+    amr = AmrScene.shock(256 // k, 4, 1200.0 / k, 40.0, 0.74, 0.55, args.seed)
+    ncells = len(amr)
+    log(f"[bench] shock AMR: {ncells} leaf cells ({time.time() - t:.1f}s)")
+    t = time.time()
+    dm = DualMesh(amr)
+    counts = dm.counts()
+    del amr
+    log(f"[bench] dual mesh: {counts} ({time.time() - t:.1f}s)")
+    t = time.time()
+    model = ExaStitchModel.from_dual_mesh(dm)
+    del dm
+    r = OWLRenderer(model, (args.mcs, args.mcs, args.mcs))
+    del model
+    free, total = torch.cuda.mem_get_info()
+    log(f"[bench] renderer + sampler build: {time.time() - t:.1f}s, {r.sampler.info()}, "
+        f"device memory in use {(total - free) / 2**30:.1f} GiB")
+    b = r.modelBounds
+    c = 0.5 * (b[:3] + b[3:])
+    R = 1200.0 / k
+    vr = r.valueRange
+    rgba = tf_rasterize(TF_SHOCK, 128)
+    os_ = 0.05
+    r.setColorMap(rgba)
+    r.setRange(vr[0], vr[1])
+    r.setOpacityScale(100.0 + np.log(os_) / np.log(1.1))
+    r.setMaxBounces(args.bounces)
+    r.setDirectionalLight(0, (0.3, 1.0, 0.5), 2.0, True)
+    W, H = 3840, 2160
+    r.resize(W, H)
+    main_cam = camera_look_at(c + np.array([-0.5, 0.45, 1.0]) * 1.9 * R, c, (0, 1, 0), 50.0, W, H)
+    r.setCamera(*main_cam)
+    rng = np.random.default_rng(args.seed)
+    views = [(W, H, main_cam)]
+    for _ in range(16):
+        az, el = float(rng.uniform(0, 2 * np.pi)), float(rng.uniform(-0.7, 0.7))
+        eye = c + 2.2 * R * np.array([np.cos(el) * np.cos(az), np.sin(el), np.cos(el) * np.sin(az)])
+        views.append((1024, 1024, camera_look_at(eye, c, (0, 1, 0), 45.0, 1024, 1024)))
+    pixels = sum(w * h for w, h, _ in views)
+    st = r.state()
+
+    def make(spp):
+        bt = RenderBatch(r)
+        ti = bt.addTransferFunction(rgba, st.tfDomain[0], st.tfDomain[1], st.opacityScale)
+        for w, h, cam in views:
+            bt.addRequest(w, h, spp, ti, cam, args.integrator)
+        return bt
+
+    t = time.time()
+    tune = make(1)
+    best, best_ms = tune.autotune((8, 16, 32), (8, 16, 32), (0, 8, 16))
+    del tune
+    log(f"[bench] autotune on the 1-spp batch: best {best} ({best_ms:.1f} ms/frame), {time.time() - t:.1f}s")
+    stream = torch.cuda.current_stream()
+
+    def timed_frame(bt):
+        bt.resetAccum()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record(stream)
+        bt.render(stream)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1)
+
+    bt = make(args.large_spp)
+    bt.setTuning(*best)
+    ms_exact = timed_frame(bt)
+    img_exact = bt.accum(0).astype(np.float64)
+    log(f"[bench] exact batch frame ({pixels} pixels x {args.large_spp} spp): {ms_exact:.0f} ms, "
+        f"{pixels * args.large_spp / (ms_exact * 1e-3) / 1e6:.0f}M samples/s")
+    small = make(1)
+    small.setTuning(*best)
+    c_exact = small.render_counted(stream)
+    # neural field retrained on this scene
+    f = NeuralField(r, levels=16, log2T=19, n_min=16, n_max=args.large_nmax, seed=args.seed)
+    t = time.time()
+    nvalid = f.makePool(args.pool, 0.7, seed=args.seed + 6)
+    log(f"[bench] pool: {nvalid} of {args.pool} samples ({time.time() - t:.1f}s)")
+    t = time.time()
+    loss = f.train(args.train_steps, batch=args.train_batch, lr=1e-2, lr_end=args.lr_end)
+    train_s = time.time() - t
+    log(f"[bench] trained {args.train_steps} steps x {args.train_batch} in {train_s:.1f}s; "
+        f"loss {loss[0]:.3g} -> {np.mean(loss[-50:]):.3g}")
+    t = time.time()
+    v = f.validate(points_per_mc=args.val_points, bound=args.val_bound * float(vr[1] - vr[0]))
+    log(f"[bench] validation ({time.time() - t:.1f}s): {v}")
+    r.setNeural(f)
+    ms_net = timed_frame(bt)
+    img_net = bt.accum(0).astype(np.float64)
+    c_net = small.render_counted(stream)
+    r.setNeural(None)
+    rmse = float(np.sqrt(np.mean((img_net - img_exact)[..., :3] ** 2))) / args.large_spp
+    free, total = torch.cuda.mem_get_info()
+    log(f"[bench] neural batch frame: {ms_net:.0f} ms; image RMSE vs exact (same seeds) {rmse:.3g}; "
+        f"network lookups {c_net['neural']}; device memory in use {(total - free) / 2**30:.1f} GiB")
+    samples = pixels * args.large_spp
+    return {
+        "metric": METRIC, "value": samples / (ms_exact * 1e-3), "unit": "samples/s", "n_gpus": 1, "steps": 1,
+        "warmup": 0, "ms_per_step": ms_exact, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32 (exact) / bf16 network", "data": "synthetic",
+        "config": {"workload": f"5-level shock-shell AMR ({ncells} leaf cells, base {256 // k}^3 x width 16, "
+                               f"radius {1200.0 / k:g}, thickness 40), dual mesh {counts['elements']} stitching "
+                               f"elements + {counts['gridlets']} gridlets, {args.mcs}^3 majorant grid; batched "
+                               f"3840x2160 + 16 x 1024^2 views, {args.large_spp} spp each, one persistent launch",
+                   "stitching_share_of_cells": counts["elements"] / ncells, "pixels": pixels,
+                   "autotuned": list(best), "exact_ms": ms_exact, "neural_ms": ms_net,
+                   "neural_samples_per_s": samples / (ms_net * 1e-3), "image_rmse_neural_vs_exact": rmse,
+                   "validation": v, "train_steps": args.train_steps, "train_seconds": train_s,
+                   "counters_exact_1spp": c_exact, "counters_neural_1spp": c_net,
+                   "device_gib_in_use": (total - free) / 2**30},
+        "roofline": None, "cpu_baseline": None,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -347,9 +483,13 @@ def main():
     ap.add_argument("--rare-batch", type=int, default=8)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["frame", "batch", "neural"], default="frame",
+    ap.add_argument("--workload", choices=["frame", "batch", "neural", "large"], default="frame",
                     help="frame: the 1080p frame (default); batch: batched requests in one launch; "
-                         "neural: exact vs neural-field element stage")
+                         "neural: exact vs neural-field element stage; large: the 400M-cell shock scene, "
+                         "batched 4K + 16 side views with and without the network")
+    ap.add_argument("--large-scale", type=int, default=1, help="large workload: base resolution and radius / k")
+    ap.add_argument("--large-spp", type=int, default=64)
+    ap.add_argument("--large-nmax", type=int, default=2048, help="large workload: finest hash-grid resolution")
     ap.add_argument("--pool", type=int, default=1 << 24)
     ap.add_argument("--train-steps", type=int, default=2000)
     ap.add_argument("--train-batch", type=int, default=1 << 18)
@@ -373,6 +513,9 @@ def main():
     dev = torch.device("cuda:0")
     if args.calibrate_copy:
         copy_calibration(args.calibrate_copy, dev)
+    if args.workload == "large":
+        print(json.dumps(large_workload(args, dev)), flush=True)
+        return
     amr, dm, model, counts = build_scene(args)
     r, rgba = setup_renderer(model, args)
     if args.workload == "batch":
