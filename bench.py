@@ -463,6 +463,60 @@ def large_workload(args, dev):
     }
 
 
+def locate_workload(r, args, dev):
+    """The sampler alone (ExaStitchSampler sample(), ExaStitchSampler.h:128-140):
+    point location + interpolation of 2^24 points per launch, uniformly random
+    over the model box (every query an independent gather) and the same points
+    in Morton order (neighbouring lanes hit neighbouring records, as samples
+    along coherent rays do).  Algorithmic bytes: the leaf record per hit
+    (BYTES) + 12 B position in + 12 B sample out."""
+    import torch
+    n = args.locate_points
+    b = r.modelBounds
+    rng = np.random.default_rng(args.seed)
+    pos = rng.uniform(b[:3], b[3:], (n, 3)).astype(np.float32)
+    # 10-bit Morton code of the position in the box
+    q = np.clip(((pos - b[:3]) / (b[3:] - b[:3]) * 1023.0).astype(np.uint64), 0, 1023)
+    code = np.zeros(n, np.uint64)
+    for bit in range(10):
+        for k in range(3):
+            code |= ((q[:, k] >> np.uint64(bit)) & np.uint64(1)) << np.uint64(3 * bit + k)
+    orders = {"uniform": pos, "morton": pos[np.argsort(code, kind="stable")]}
+    out = torch.empty((n, 3), dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream()
+    res = {}
+    for name, p in orders.items():
+        d_pos = torch.from_numpy(np.ascontiguousarray(p)).to(dev)
+        for _ in range(args.warmup):
+            r.sampler.locate(d_pos, out, stream)
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        torch.cuda.synchronize()
+        ev[0].record(stream)
+        for _ in range(args.steps):
+            r.sampler.locate(d_pos, out, stream)
+        ev[1].record(stream)
+        torch.cuda.synchronize()
+        ms = ev[0].elapsed_time(ev[1]) / args.steps
+        c = r.sampler.locate_counted(d_pos, out, stream)
+        alg = sum(BYTES[k] * c[k] for k in ("gridlet", "tet", "pyr", "wedge", "hex")) + 24 * n
+        gbs = alg / (ms * 1e-3) / 1e9
+        res[name] = {"ms_per_launch": ms, "queries_per_s": n / (ms * 1e-3), "algorithmic_bytes": alg,
+                     "achieved_GBps": gbs, "frac": gbs / HBM_PEAK_GBS, "counters": c}
+        log(f"[bench] locate {name}: {ms:.3f} ms per {n} queries = {n / (ms * 1e-3) / 1e9:.2f}G queries/s, "
+            f"{gbs:.0f} GB/s algorithmic ({100 * gbs / HBM_PEAK_GBS:.1f}% of HBM peak); {c}")
+    best = max(res.values(), key=lambda v: v["queries_per_s"])
+    return {
+        "metric": METRIC, "value": best["queries_per_s"], "unit": "point queries/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": best["ms_per_launch"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": f"point location + interpolation of {n} points per launch in the 3-level noise "
+                               f"scene (tables accelerator)", "orders": res},
+        "roofline": {"bound": "hbm", "achieved": best["achieved_GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": best["frac"], "traffic": None, "kernel": "kLocate"},
+        "cpu_baseline": None,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -483,7 +537,8 @@ def main():
     ap.add_argument("--rare-batch", type=int, default=8)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["frame", "batch", "neural", "large"], default="frame",
+    ap.add_argument("--locate-points", type=int, default=1 << 24)
+    ap.add_argument("--workload", choices=["frame", "batch", "neural", "large", "locate"], default="frame",
                     help="frame: the 1080p frame (default); batch: batched requests in one launch; "
                          "neural: exact vs neural-field element stage; large: the 400M-cell shock scene, "
                          "batched 4K + 16 side views with and without the network")
@@ -523,6 +578,9 @@ def main():
         return
     if args.workload == "neural":
         print(json.dumps(neural_workload(r, args, dev)), flush=True)
+        return
+    if args.workload == "locate":
+        print(json.dumps(locate_workload(r, args, dev)), flush=True)
         return
     stream = torch.cuda.current_stream()
     fb = torch.zeros((args.height, args.width), dtype=torch.int32, device=dev)

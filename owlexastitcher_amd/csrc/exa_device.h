@@ -385,6 +385,81 @@ EXA_HD bool intersectNewton(float &value, vec3f P, const float4 *V) {
   return false;
 }
 
+// intersectNewton<N> for a vertex count known only at run time (5, 6 or 8):
+// the shape functions are evaluated for that type into 8 slots and the
+// iteration is shared, so the lanes of a wave holding different element types
+// run one Newton loop together instead of one per type.  The sums stop at
+// nv terms, so every value matches intersectNewton<nv> bit for bit.
+template <int N>
+EXA_HD void shapeSlots(const float *pc, float w[8], float dr[8], float ds[8], float dt[8]) {
+  float sw[N], d[3 * N];
+  ShapeFn<N>::sf(pc, sw);
+  ShapeFn<N>::df(pc, d);
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    w[i] = i < N ? sw[i] : 0.f;
+    dr[i] = i < N ? d[i] : 0.f;
+    ds[i] = i < N ? d[i + N] : 0.f;
+    dt[i] = i < N ? d[i + 2 * N] : 0.f;
+  }
+}
+EXA_HD void shapeAny(int nv, const float *pc, float w[8], float dr[8], float ds[8], float dt[8]) {
+  if (nv == 8) shapeSlots<8>(pc, w, dr, ds, dt);
+  else if (nv == 6) shapeSlots<6>(pc, w, dr, ds, dt);
+  else shapeSlots<5>(pc, w, dr, ds, dt);
+}
+
+EXA_HD bool intersectNewtonAny(float &value, vec3f P, const float4 *V, int nv) {
+  const float determinantTolerance = 1e-6f;
+  float pc[3] = {.5f, .5f, .5f};
+  float w[8], dr[8], ds[8], dt[8];
+  bool converged = false;
+  for (int iteration = 0; !converged && (iteration < 10); iteration++) {
+    shapeAny(nv, pc, w, dr, ds, dt);
+    vec3f fcol = mk3s(0.f), rcol = mk3s(0.f), scol = mk3s(0.f), tcol = mk3s(0.f);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      if (i < nv) {
+        const vec3f pt = xyz(V[i]);
+        fcol = fcol + pt * w[i];
+        rcol = rcol + pt * dr[i];
+        scol = scol + pt * ds[i];
+        tcol = tcol + pt * dt[i];
+      }
+    }
+    fcol = fcol - P;
+    const float d = det3(rcol, scol, tcol);
+    if (fabsf(d) < determinantTolerance) return false;
+    const float d0 = det3(fcol, scol, tcol) / d;
+    const float d1 = det3(rcol, fcol, tcol) / d;
+    const float d2 = det3(rcol, scol, fcol) / d;
+    pc[0] = pc[0] - d0;
+    pc[1] = pc[1] - d1;
+    pc[2] = pc[2] - d2;
+    const float kConverged = 0x1.a36e30p-14f, kDiverged = 1e6f;
+    if (fabsf(d0) < kConverged && fabsf(d1) < kConverged && fabsf(d2) < kConverged) {
+      converged = true;
+    } else if (fabsf(pc[0]) > kDiverged || fabsf(pc[1]) > kDiverged || fabsf(pc[2]) > kDiverged) {
+      return false;
+    }
+  }
+  if (!converged) return false;
+  const float lowerlimit = (float)(0.0 - 1e-6);
+  const float upperlimit = (float)(1.0 + 1e-6);
+  bool inside = pc[0] >= lowerlimit && pc[0] <= upperlimit && pc[1] >= lowerlimit && pc[1] <= upperlimit &&
+                pc[2] >= lowerlimit && pc[2] <= upperlimit;
+  if (nv == 6) inside = inside && (pc[0] + pc[1] <= upperlimit);
+  if (inside) {
+    float val = 0.f;
+#pragma unroll
+    for (int i = 0; i < 8; i++)
+      if (i < nv) val += w[i] * V[i].w;
+    value = val;
+    return true;
+  }
+  return false;
+}
+
 // programs/ExaStitchSampler.cu:133-161 StitchGeomIsect dispatch on the
 // number of valid indices in the element's int[8] row.
 EXA_HD bool intersectElement(float &value, vec3f P, const int32_t *__restrict__ idx8,

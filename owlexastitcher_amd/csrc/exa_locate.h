@@ -90,9 +90,7 @@ EXA_DEV int testElement(float &value, vec3f P, const int32_t *__restrict__ idx8,
   }
   bool hit;
   if (nv == 4) hit = intersectTet(value, P, v[0], v[1], v[2], v[3]);
-  else if (nv == 5) hit = intersectNewton<5>(value, P, v);
-  else if (nv == 6) hit = intersectNewton<6>(value, P, v);
-  else if (nv == 8) hit = intersectNewton<8>(value, P, v);
+  else if (nv == 5 || nv == 6 || nv == 8) hit = intersectNewtonAny(value, P, v, nv);
   else hit = false;
   if (!hit) return -1;
   return nv == 4 ? 0 : (nv == 5 ? 1 : (nv == 6 ? 2 : 3));
@@ -111,9 +109,7 @@ EXA_DEV int testElementRecord(float &value, vec3f P, int nv, const float4 *__res
   if (nv > 6) { v[6] = rec[6]; v[7] = rec[7]; }
   bool hit;
   if (nv == 4) hit = intersectTet(value, P, v[0], v[1], v[2], v[3]);
-  else if (nv == 5) hit = intersectNewton<5>(value, P, v);
-  else if (nv == 6) hit = intersectNewton<6>(value, P, v);
-  else if (nv == 8) hit = intersectNewton<8>(value, P, v);
+  else if (nv == 5 || nv == 6 || nv == 8) hit = intersectNewtonAny(value, P, v, nv);
   else hit = false;
   if (!hit) return -1;
   return nv == 4 ? 0 : (nv == 5 ? 1 : (nv == 6 ? 2 : 3));
@@ -276,7 +272,9 @@ EXA_DEV Sample locateTablesElements(const SamplerView &sv, const Culling &cu, co
   }
   const int64_t bin = ((int64_t)b[2] * sv.binDims[1] + b[1]) * sv.binDims[0] + b[0];
   const uint32_t k0 = sv.eStart[bin], k1 = sv.eStart[bin + 1];
-  // candidates in ID order, four records per batch of independent loads
+  // candidates in ID order, four records per batch of independent loads;
+  // the AABB tests of a batch give a mask, then one element test per set bit
+  // (lanes test their own candidates together, whatever the element types)
   for (uint32_t k = k0; k < k1; k += 4) {
     float4 lo[4], hi[4];
 #pragma unroll
@@ -285,16 +283,22 @@ EXA_DEV Sample locateTablesElements(const SamplerView &sv, const Culling &cu, co
         lo[j] = sv.eCand[2 * (size_t)(k + j)];
         hi[j] = sv.eCand[2 * (size_t)(k + j) + 1];
       }
+    unsigned hits = 0;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      if (k + j >= k1) break;
-      if (!(p.x >= lo[j].x && p.y >= lo[j].y && p.z >= lo[j].z && p.x <= hi[j].x && p.y <= hi[j].y &&
-            p.z <= hi[j].z))
-        continue;
-      const int32_t e = __float_as_int(lo[j].w);
+    for (int j = 0; j < 4; ++j)
+      if (k + j < k1 && p.x >= lo[j].x && p.y >= lo[j].y && p.z >= lo[j].z && p.x <= hi[j].x &&
+          p.y <= hi[j].y && p.z <= hi[j].z)
+        hits |= 1u << j;
+    while (hits) {
+      const int j = __builtin_ctz(hits);
+      hits &= hits - 1;
+      // the candidate's id and vertex count (selects, not an indexed array)
+      const float ew = j == 0 ? lo[0].w : (j == 1 ? lo[1].w : (j == 2 ? lo[2].w : lo[3].w));
+      const float nw = j == 0 ? hi[0].w : (j == 1 ? hi[1].w : (j == 2 ? hi[2].w : hi[3].w));
+      const int32_t e = __float_as_int(ew);
       if (cu.elemMaxOp && cu.elemMaxOp[e] == 0.f) continue;
       float v;
-      const int t = testElementRecord(v, p, __float_as_int(hi[j].w), sv.eVerts + 8 * (int64_t)e);
+      const int t = testElementRecord(v, p, __float_as_int(nw), sv.eVerts + 8 * (int64_t)e);
       if (t >= 0) {
         s.primID = e;
         s.cellID = -1;
