@@ -811,12 +811,24 @@ int exa_sampler_compute_max_opacities(exa_sampler *s, const float *rgba, int n, 
     s->computeMaxOpacities(rgba, n, lo, hi, (hipStream_t)stream);
   });
 }
+// Point location of n queries: gridlet stage, then the element stage over a
+// queue of the queries left (stream-ordered scratch of n + 1 uint32, so calls
+// on different streams do not share it); one pass per query beyond 2^32 - 2.
+static void locateTwoStage(const exa_sampler *s, const float *d_pos, int64_t n, Sample *d_out,
+                           unsigned long long *counters, hipStream_t st) {
+  if (n <= 0) return;
+  uint32_t *queue = nullptr;
+  if (n < (int64_t)0xFFFFFFFEll) HIP_CHECK(hipMallocAsync((void **)&queue, (size_t)(n + 1) * sizeof(uint32_t), st));
+  launchLocate(s->view(), s->accel, d_pos, n, d_out, counters, st, queue);
+  const hipError_t e = hipGetLastError();
+  if (queue) HIP_CHECK(hipFreeAsync(queue, st));
+  HIP_CHECK(e);
+}
 int exa_sampler_locate(const exa_sampler *s, const float *d_pos, int64_t n, exa_sample *d_out, void *stream) {
   return guarded([&] {
     require(s && (n == 0 || (d_pos && d_out)) && n >= 0, "invalid arguments");
     if (!s->built) throw Status(EXA_ERR_STATE, "sampler not built");
-    launchLocate(s->view(), s->accel, d_pos, n, (Sample *)d_out, nullptr, (hipStream_t)stream);
-    HIP_CHECK(hipGetLastError());
+    locateTwoStage(s, d_pos, n, (Sample *)d_out, nullptr, (hipStream_t)stream);
   });
 }
 int exa_sampler_locate_counted(const exa_sampler *s, const float *d_pos, int64_t n, exa_sample *d_out,
@@ -826,8 +838,7 @@ int exa_sampler_locate_counted(const exa_sampler *s, const float *d_pos, int64_t
     if (!s->built) throw Status(EXA_ERR_STATE, "sampler not built");
     hipStream_t st = (hipStream_t)stream;
     HIP_CHECK(hipMemsetAsync(s->counters.ptr, 0, kNumCounters * sizeof(unsigned long long), st));
-    launchLocate(s->view(), s->accel, d_pos, n, (Sample *)d_out, s->counters.ptr, st);
-    HIP_CHECK(hipGetLastError());
+    locateTwoStage(s, d_pos, n, (Sample *)d_out, s->counters.ptr, st);
     unsigned long long h[kNumCounters];
     HIP_CHECK(hipMemcpyAsync(h, s->counters.ptr, sizeof(h), hipMemcpyDeviceToHost, st));
     HIP_CHECK(hipStreamSynchronize(st));

@@ -79,8 +79,10 @@ void launchMapMaxOpacity(const float2 *ranges, int64_t n, const float4 *cmap, in
 void launchElementMaxOpacity(const float4 *vertices, const int32_t *indices, int64_t n,
                              const float4 *cmap, int numColors, float x0, float x1, float *out,
                              hipStream_t s);
+// queue: n + 1 uint32 of device scratch for the two-stage scheme, or nullptr
+// for one pass per query
 void launchLocate(const SamplerView &sv, int accel, const float *pos, int64_t n, Sample *out,
-                  unsigned long long *counters, hipStream_t s);
+                  unsigned long long *counters, hipStream_t s, uint32_t *queue = nullptr);
 void launchTestSample(const SamplerView &sv, const float *pos, const int32_t *primIDs, int64_t n,
                       Sample *out, hipStream_t s);
 

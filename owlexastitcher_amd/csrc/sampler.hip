@@ -4,6 +4,7 @@
 //   * max-opacity mapping        (Grid.cu:687-719, ExaStitchSampler.cu:144-227)
 //   * batched point location     (ExaStitchSampler.h:108-140 + the OptiX
 //                                 programs in programs/ExaStitchSampler.cu)
+#include <algorithm>
 #include "exa_locate.h"
 
 namespace exa {
@@ -252,6 +253,67 @@ __global__ __launch_bounds__(kLocateBlock) void kLocate(SamplerView sv, const fl
   if (kCount) atomicAdd(&counters[kind < 0 ? kCntMiss : kCntGridletHit + kind], 1ull);
 }
 
+// Batched point location in two stages.  Stage 1 answers every query that a
+// gridlet holds (the owner tables' speculative lookup with the tables
+// accelerator) and appends the others to a queue, one atomic per wave; stage
+// 2 runs the element stage over the queue, so its waves are full of element
+// queries instead of carrying the ~20% of them spread over every wave.
+// queue[0] counts the entries queue[1..].
+template <bool kCount, int kAccel>
+__global__ __launch_bounds__(kLocateBlock) void kLocateGridletStage(SamplerView sv, const float *__restrict__ pos,
+                                                                    int64_t n, Sample *__restrict__ out,
+                                                                    uint32_t *__restrict__ queue,
+                                                                    unsigned long long *counters) {
+  __shared__ int32_t stack[kAccel == kAccelBvh ? kStackDepth * kLocateBlock : 1];
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  bool pending = false;
+  if (i < n) {
+    const vec3f p = mk3(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
+    const Culling cu = cullingOf(sv);
+    Sample s;
+    if (kAccel == kAccelTables) {
+      bool cachedHit;
+      s = lookupTablesGridlets(sv, cu, p, -1, cachedHit);
+    } else {
+      s = locateBvhGridlets(sv, cu, p, stack + threadIdx.x, kLocateBlock);
+    }
+    if (s.primID >= 0) {
+      out[i] = s;
+      if (kCount) atomicAdd(&counters[kCntGridletHit], 1ull);
+    } else {
+      pending = true;
+    }
+  }
+  const uint64_t mask = __ballot(pending);
+  if (mask == 0) return;
+  const int lane = (int)__lane_id();
+  const int leader = __ffsll((unsigned long long)mask) - 1;
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(&queue[0], (uint32_t)__popcll(mask));
+  base = __shfl(base, leader);
+  if (pending) queue[1 + base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] = (uint32_t)i;
+}
+
+template <bool kCount, int kAccel>
+__global__ __launch_bounds__(kLocateBlock) void kLocateElementStage(SamplerView sv, const float *__restrict__ pos,
+                                                                     Sample *__restrict__ out,
+                                                                     const uint32_t *__restrict__ queue,
+                                                                     unsigned long long *counters) {
+  __shared__ int32_t stack[kAccel == kAccelBvh ? kStackDepth * kLocateBlock : 1];
+  const uint32_t count = queue[0];
+  const Culling cu = cullingOf(sv);
+  for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < count; q += gridDim.x * blockDim.x) {
+    const uint32_t i = queue[1 + q];
+    const vec3f p = mk3(pos[3 * (int64_t)i], pos[3 * (int64_t)i + 1], pos[3 * (int64_t)i + 2]);
+    int kind;
+    const Sample s = locateElements<kAccel>(sv, cu, p, stack + threadIdx.x, kLocateBlock, kind);
+    out[i] = s;
+    if (kCount) atomicAdd(&counters[kind < 0 ? kCntMiss : kCntGridletHit + kind], 1ull);
+  }
+}
+
+__global__ void kZeroU32(uint32_t *p) { *p = 0u; }
+
 __global__ void kTestSample(SamplerView sv, const float *__restrict__ pos, const int32_t *__restrict__ prim,
                             int64_t n, Sample *__restrict__ out) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -300,16 +362,32 @@ void launchElementMaxOpacity(const float4 *vertices, const int32_t *indices, int
 }
 
 void launchLocate(const SamplerView &sv, int accel, const float *pos, int64_t n, Sample *out,
-                  unsigned long long *counters, hipStream_t s) {
+                  unsigned long long *counters, hipStream_t s, uint32_t *queue) {
   if (n <= 0) return;
   const dim3 g(blocksFor(n, kLocateBlock)), b(kLocateBlock);
-  if (accel == kAccelBvh) {
-    if (counters) hipLaunchKernelGGL((kLocate<true, kAccelBvh>), g, b, 0, s, sv, pos, n, out, counters);
-    else hipLaunchKernelGGL((kLocate<false, kAccelBvh>), g, b, 0, s, sv, pos, n, out, counters);
-  } else {
-    if (counters) hipLaunchKernelGGL((kLocate<true, kAccelTables>), g, b, 0, s, sv, pos, n, out, counters);
-    else hipLaunchKernelGGL((kLocate<false, kAccelTables>), g, b, 0, s, sv, pos, n, out, counters);
+  if (!queue) {  // one pass: gridlet and element stage per thread
+    if (accel == kAccelBvh) {
+      if (counters) hipLaunchKernelGGL((kLocate<true, kAccelBvh>), g, b, 0, s, sv, pos, n, out, counters);
+      else hipLaunchKernelGGL((kLocate<false, kAccelBvh>), g, b, 0, s, sv, pos, n, out, counters);
+    } else {
+      if (counters) hipLaunchKernelGGL((kLocate<true, kAccelTables>), g, b, 0, s, sv, pos, n, out, counters);
+      else hipLaunchKernelGGL((kLocate<false, kAccelTables>), g, b, 0, s, sv, pos, n, out, counters);
+    }
+    return;
   }
+  // two stages over a queue of n + 1 entries; stage 2 walks the queue with at
+  // most 4096 blocks
+  const dim3 g2(std::min<unsigned>(g.x, 4096u));
+  hipLaunchKernelGGL(kZeroU32, dim3(1), dim3(1), 0, s, queue);
+#define EXA_STAGES(C, A)                                                                                  \
+  hipLaunchKernelGGL((kLocateGridletStage<C, A>), g, b, 0, s, sv, pos, n, out, queue, counters);          \
+  hipLaunchKernelGGL((kLocateElementStage<C, A>), g2, b, 0, s, sv, pos, out, queue, counters)
+  if (accel == kAccelBvh) {
+    if (counters) { EXA_STAGES(true, kAccelBvh); } else { EXA_STAGES(false, kAccelBvh); }
+  } else {
+    if (counters) { EXA_STAGES(true, kAccelTables); } else { EXA_STAGES(false, kAccelTables); }
+  }
+#undef EXA_STAGES
 }
 
 void launchTestSample(const SamplerView &sv, const float *pos, const int32_t *primIDs, int64_t n, Sample *out,
