@@ -42,7 +42,6 @@ constexpr int kPixelTile = 8;  // 8x8 pixels per 64-lane grab
 enum Stage : int32_t {
   // tracking pass
   kStStep = 0,  // free-flight step inside the macrocell
-  kStCellEnd,   // DDA step to the next macrocell
   kStCellBegin, // enter a macrocell
   // point location and resolution
   kStAtSample,  // collision position ready for point location
@@ -122,25 +121,35 @@ EXA_DEV void endSample(PathLane &L, const RenderParams &P, float4 c, LocalCounte
   L.st = kStGrab;
 }
 
-// tracking pass transition: Woodcock / ratio-tracking free flight and dda3
+// tracking pass: Woodcock / ratio-tracking free flights and dda3 steps.
+// One iteration takes one free flight in the current macrocell and, when it
+// leaves the cell, the DDA step into the next one and that cell's set-up, so
+// every tracking lane runs the same code each iteration (the per-lane order
+// of random draws and arithmetic is that of sampleInteraction).
 template <bool kCount>
-EXA_DEV void trackingTransition(PathLane &L, const RenderParams &P, const GridView &gv, LocalCounters &cnt) {
-  if (L.st == kStStep) {
-    if (L.majorant <= 0.f || !trackingStep(L.t, L.majorant, P.opacityScale, L.t1, L.rng)) {
-      L.st = kStCellEnd;
-    } else {
+EXA_DEV void cellBegin(PathLane &L, const GridView &gv, LocalCounters &cnt) {
+  L.t1 = segmentCellExit(L.S);
+  if (kCount) cnt.c[kCntDdaCells]++;
+  L.majorant = gv.majorants[segmentCellID(gv, L.S)];
+  L.t = L.S.t0;
+  L.primID = -1;
+}
+
+template <bool kCount>
+EXA_DEV void trackingRun(PathLane &L, const RenderParams &P, const GridView &gv, int steps, LocalCounters &cnt) {
+  if (L.st == kStCellBegin) {
+    cellBegin<kCount>(L, gv, cnt);
+    L.st = kStStep;
+  }
+  for (int n = 0; n < steps && L.st == kStStep; ++n) {
+    if (L.majorant > 0.f && trackingStep(L.t, L.majorant, P.opacityScale, L.t1, L.rng)) {
       L.pos = L.S.org + L.S.dir * L.t;
       L.st = kStAtSample;
+    } else if (segmentNextCell(gv, L.S, L.t1)) {
+      cellBegin<kCount>(L, gv, cnt);
+    } else {
+      L.st = kStSegEnd;
     }
-  } else if (L.st == kStCellEnd) {
-    L.st = segmentNextCell(gv, L.S, L.t1) ? kStCellBegin : kStSegEnd;
-  } else {
-    L.t1 = segmentCellExit(L.S);
-    if (kCount) cnt.c[kCntDdaCells]++;
-    L.majorant = gv.majorants[segmentCellID(gv, L.S)];
-    L.t = L.S.t0;
-    L.primID = -1;
-    L.st = kStStep;
   }
 }
 
@@ -340,10 +349,7 @@ kRenderPersistent(RenderParams P, SamplerView sv, GridView gv, PersistParams pp,
     }
 
     // tracking pass
-    for (int n = 0; n < pp.ctrlSteps; ++n) {
-      if (!isTracking(L.st)) break;
-      trackingTransition<kCount>(L, Pl, gvl, cnt);
-    }
+    trackingRun<kCount>(L, Pl, gvl, pp.ctrlSteps, cnt);
 
     // gridlet pass (cached gridlet first)
     Sample smp{-1, -1, 0.f};
