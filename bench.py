@@ -532,9 +532,9 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--accel", choices=["tables", "bvh"], default="tables")
     ap.add_argument("--scheme", choices=["persistent", "megakernel"], default="persistent")
-    ap.add_argument("--ctrl-steps", type=int, default=16)
+    ap.add_argument("--ctrl-steps", type=int, default=6)
     ap.add_argument("--elem-batch", type=int, default=16)
-    ap.add_argument("--rare-batch", type=int, default=8)
+    ap.add_argument("--rare-batch", type=int, default=16)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--locate-points", type=int, default=1 << 24)

@@ -259,10 +259,11 @@ int exa_renderer_set_accel(exa_renderer *r, int accel);
    EXA_RENDER_MEGAKERNEL: one thread per pixel, 16x16 tiles.
    EXA_RENDER_PERSISTENT (default): persistent lanes that take pixels from a
    work counter and run their paths as a state machine, in per-wave passes.
-   Tuning: ctrlSteps = free-flight / macrocell transitions per lane and pass
+   Tuning: ctrlSteps = tracking iterations (one free flight, and the DDA
+   step into the next macrocell when it leaves the cell) per lane and pass
    (>= 1); elemBatch and rareBatch in [0, 64] = wave fraction (x/64) of live
    lanes that must wait for an element test, resp. for per-segment work,
-   before that pass runs.  Defaults 16, 16, 8. */
+   before that pass runs.  Defaults 6, 16, 16. */
 #define EXA_RENDER_MEGAKERNEL 0
 #define EXA_RENDER_PERSISTENT 1
 int exa_renderer_set_scheme(exa_renderer *r, int scheme);

@@ -146,9 +146,9 @@ constexpr int kMaxBatchRequests = 1024;
 
 // scheduling knobs of the persistent renderer (results do not depend on them)
 struct PersistTuning {
-  int32_t ctrlSteps = 16;  // tracking transitions per lane and round, >= 1
+  int32_t ctrlSteps = 6;   // tracking iterations per lane and round, >= 1
   int32_t elemBatch = 16;  // element pass when waiting lanes >= elemBatch/64 of live lanes
-  int32_t rareBatch = 8;   // per-segment pass when waiting lanes >= rareBatch/64 of live lanes
+  int32_t rareBatch = 16;  // per-segment pass when waiting lanes >= rareBatch/64 of live lanes
 };
 // work: one device uint32 (reset by the launch itself)
 struct NeuralView;
