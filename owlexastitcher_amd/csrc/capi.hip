@@ -159,6 +159,7 @@ struct exa_sampler {
   DeviceBuffer<int32_t> gRefs, eRefs;
   int gDepth = 0, eDepth = 0;
   LocateTables tabMeta;  // sizes/offsets only (vectors are released after upload)
+  float bandMargin = 0.00390625f, fastMargin = 0.0078125f;  // gridletMargins of the model
   DeviceBuffer<uint32_t> tGStart, tEStart;
   DeviceBuffer<int32_t> tGList, tGTop, tGOwner;
   DeviceBuffer<float4> tECand, tEVerts;
@@ -206,6 +207,8 @@ struct exa_sampler {
       v.binDims[k] = tabMeta.binDims[k];
     }
     v.binInv = tabMeta.binInv;
+    v.bandMargin = bandMargin;
+    v.fastMargin = fastMargin;
     return v;
   }
 
@@ -288,6 +291,7 @@ struct exa_sampler {
       throw Status(EXA_ERR_INVALID, "more than 2^31 primitives");
     numElements = m.numElements();
     numGridlets = (int64_t)m.gridlets.size();
+    gridletMargins(m, bandMargin, fastMargin);
     vertices.upload((const float4 *)m.vertices.data(), m.vertices.size() / 4, s);
     indices.upload(m.indices.data(), m.indices.size(), s);
     gridlets.upload(m.gridlets.data(), m.gridlets.size(), s);

@@ -133,11 +133,12 @@ EXA_HD void gridletBounds(const Gridlet &g, vec3f &lo, vec3f &hi) {
 }
 
 // Returns 0 = miss, 1 = hit on a cell boundary band, 2 = hit strictly inside
-// its cell (every fractional coordinate in [2^-8, 1-2^-8]); a band-2 hit is
-// the unique accepting gridlet for that point (dual cells tile space), which
-// lets point location stop early.
+// its cell (every fractional coordinate in [margin, 1-margin], margin above
+// the rounding of the cell coordinates); a band-2 hit is the unique accepting
+// gridlet for that point (dual cells tile space), which lets point location
+// stop early.
 EXA_HD int intersectGridlet(float &value, int &cellID, vec3f pos, const Gridlet &g,
-                            const float *__restrict__ scalars) {
+                            const float *__restrict__ scalars, float margin = 0.00390625f) {
   vec3f lo, hi;
   gridletBounds(g, lo, hi);
   if (!(pos.x >= lo.x && pos.y >= lo.y && pos.z >= lo.z && pos.x <= hi.x && pos.y <= hi.y &&
@@ -173,7 +174,7 @@ EXA_HD int intersectGridlet(float &value, int &cellID, vec3f pos, const Gridlet 
   const float f5678 = lerpf(f56, f78, frac.y);
   value = lerpf(f1234, f5678, frac.z);
   cellID = iz * g.dims[1] * g.dims[0] + iy * g.dims[0] + ix;
-  const float m0 = 0.00390625f, m1 = 0.99609375f;
+  const float m0 = margin, m1 = 1.f - margin;
   const bool interior = frac.x >= m0 && frac.x <= m1 && frac.y >= m0 && frac.y <= m1 &&
                         frac.z >= m0 && frac.z <= m1 && ix < g.dims[0] && iy < g.dims[1] &&
                         iz < g.dims[2];

@@ -152,10 +152,18 @@ struct LocateTables {
   int64_t levelOffset[kMaxLevels + 1] = {};
   std::vector<uint32_t> gStart;  // per macrocell (all levels concatenated) + 1
   std::vector<int32_t> gList;
-  // Dual-cell owner tables (empty when a coordinate reaches 8192, where the
-  // interior test below loses its rounding margin): per macrocell the index
-  // of its owner block or -1; per block, for each of its 8^3 dual cells, the
-  // lowest-ID gridlet whose cell there has 8 non-NaN corners, or -1.
+  // Rounding margins of the gridlet tests, from the largest coordinate
+  // magnitude 2^e of the model (any p below 2^e is computed to within
+  // 2^(e-24) in level-L cell units): a gridlet hit whose fractional cell
+  // coordinates keep bandMargin = 2^(e-21) from every face is the unique
+  // accepting gridlet; the owner tables answer points that keep
+  // fastMargin = 2^(e-20) from the faces of their level's dual cells.
+  float bandMargin = 0.00390625f;
+  float fastMargin = 0.0078125f;
+  // Dual-cell owner tables (empty when fastMargin would exceed 1/16): per
+  // macrocell the index of its owner block or -1; per block, for each of its
+  // 8^3 dual cells, the lowest-ID gridlet whose cell there has 8 non-NaN
+  // corners, or -1.
   std::vector<int32_t> gTop;
   std::vector<int32_t> gOwner;
   float binLo[3] = {0, 0, 0};
@@ -168,6 +176,8 @@ struct LocateTables {
   std::vector<float> eVerts;
 };
 void buildLocateTables(const HostModel &m, LocateTables &t);
+// LocateTables::bandMargin / fastMargin of a model
+void gridletMargins(const HostModel &m, float &bandMargin, float &fastMargin);
 void elementBoxes(const std::vector<int32_t> &indices, const std::vector<float> &vertices,
                   std::vector<float> &boxes);
 

@@ -38,6 +38,8 @@ struct SamplerView {
   float binLo[3];
   float binInv;
   int32_t binDims[3];
+  float bandMargin;  // LocateTables::bandMargin / fastMargin
+  float fastMargin;
 };
 
 enum Accel { kAccelTables = 0, kAccelBvh = 1 };

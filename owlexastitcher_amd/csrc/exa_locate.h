@@ -135,13 +135,13 @@ EXA_DEV Sample locateTablesGridlets(const SamplerView &sv, const Culling &cu, co
       continue;
     const int32_t idx = sv.levelOffset[L] + ((int32_t)mzf * D[1] + (int32_t)myf) * D[0] + (int32_t)mxf;
     if (sv.gTop) {
-      // Owner-table fast path for points at least 2^-7 of a cell away from
-      // every dual-cell face of this level (coordinates < 8192, so the
-      // gridlet test sees fractions in [2^-8, 1-2^-8], a band-2 hit).  There
-      // the lowest-ID accepting gridlet of the level is the cell's owner.
+      // Owner-table fast path for points at least fastMargin of a cell away
+      // from every dual-cell face of this level (so the gridlet test sees
+      // fractions at least bandMargin inside, a band-2 hit).  There the
+      // lowest-ID accepting gridlet of the level is the cell's owner.
       const float cx = floorf(gx), cy = floorf(gy), cz = floorf(gz);
       const float fx = gx - cx, fy = gy - cy, fz = gz - cz;
-      const float m0 = 0.0078125f, m1 = 0.9921875f;
+      const float m0 = sv.fastMargin, m1 = 1.f - sv.fastMargin;
       if (fx >= m0 && fx <= m1 && fy >= m0 && fy <= m1 && fz >= m0 && fz <= m1) {
         const int32_t blk = sv.gTop[idx];
         if (blk < 0) continue;
@@ -151,7 +151,7 @@ EXA_DEV Sample locateTablesGridlets(const SamplerView &sv, const Culling &cu, co
         if (!(cu.gridletMaxOp && cu.gridletMaxOp[g] == 0.f)) {
           float v;
           int cid;
-          if (intersectGridlet(v, cid, p, sv.gridlets[g], sv.gridletScalars) == 2) {
+          if (intersectGridlet(v, cid, p, sv.gridlets[g], sv.gridletScalars, sv.bandMargin) == 2) {
             s.primID = g;
             s.cellID = cid;
             s.value = v;
@@ -168,7 +168,7 @@ EXA_DEV Sample locateTablesGridlets(const SamplerView &sv, const Culling &cu, co
       if (cu.gridletMaxOp && cu.gridletMaxOp[g] == 0.f) continue;
       float v;
       int cid;
-      const int r = intersectGridlet(v, cid, p, sv.gridlets[g], sv.gridletScalars);
+      const int r = intersectGridlet(v, cid, p, sv.gridlets[g], sv.gridletScalars, sv.bandMargin);
       if (r) {
         best = g;
         s.primID = g;
@@ -216,7 +216,7 @@ EXA_DEV Sample lookupTablesGridlets(const SamplerView &sv, const Culling &cu, co
       continue;
     const float cx = floorf(gx), cy = floorf(gy), cz = floorf(gz);
     const float fx = gx - cx, fy = gy - cy, fz = gz - cz;
-    const float m0 = 0.0078125f, m1 = 0.9921875f;
+    const float m0 = sv.fastMargin, m1 = 1.f - sv.fastMargin;
     if (!(fx >= m0 && fx <= m1 && fy >= m0 && fy <= m1 && fz >= m0 && fz <= m1)) {
       if (undecided == kFastLevels) undecided = L;
       continue;
@@ -233,7 +233,7 @@ EXA_DEV Sample lookupTablesGridlets(const SamplerView &sv, const Culling &cu, co
   if (cached >= 0) {
     float v;
     int cid;
-    if (intersectGridlet(v, cid, p, cg, sv.gridletScalars)) {
+    if (intersectGridlet(v, cid, p, cg, sv.gridletScalars, sv.bandMargin)) {
       cachedHit = true;
       return Sample{cached, cid, v};
     }
@@ -252,7 +252,7 @@ EXA_DEV Sample lookupTablesGridlets(const SamplerView &sv, const Culling &cu, co
   if (!(cu.gridletMaxOp && cu.gridletMaxOp[g] == 0.f)) {
     float v;
     int cid;
-    if (intersectGridlet(v, cid, p, sv.gridlets[g], sv.gridletScalars) == 2) return Sample{g, cid, v};
+    if (intersectGridlet(v, cid, p, sv.gridlets[g], sv.gridletScalars, sv.bandMargin) == 2) return Sample{g, cid, v};
   }
   return locateTablesGridlets(sv, cu, p);
 }
@@ -324,7 +324,7 @@ EXA_DEV Sample locateBvhGridlets(const SamplerView &sv, const Culling &cu, const
       if (cu.gridletMaxOp && cu.gridletMaxOp[g] == 0.f) continue;
       float v;
       int cid;
-      const int r = intersectGridlet(v, cid, p, sv.gridlets[g], sv.gridletScalars);
+      const int r = intersectGridlet(v, cid, p, sv.gridlets[g], sv.gridletScalars, sv.bandMargin);
       if (r) {
         best = g;
         s.primID = g;
@@ -402,7 +402,7 @@ EXA_DEV Sample testSampleGridlet(const SamplerView &sv, const vec3f p, int32_t p
   if (primID < 0) return s;
   float v;
   int cid;
-  if (intersectGridlet(v, cid, p, sv.gridlets[primID], sv.gridletScalars)) {
+  if (intersectGridlet(v, cid, p, sv.gridlets[primID], sv.gridletScalars, sv.bandMargin)) {
     s.primID = primID;
     s.cellID = cid;
     s.value = v;

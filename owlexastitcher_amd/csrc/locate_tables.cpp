@@ -25,9 +25,7 @@ static inline int32_t binOf(float x, float lo, float inv, int32_t dims) {
 // owner of a dual cell is the first gridlet (in ID order) that has a valid
 // cell there; blocks are allocated for macrocells that hold any valid cell.
 static void buildOwnerTables(const HostModel &m, LocateTables &t) {
-  for (const Gridlet &g : m.gridlets)
-    for (int k = 0; k < 3; ++k)
-      if ((int64_t)(g.lower[k] + g.dims[k] + 1) << g.level >= 8192) return;
+  if (t.fastMargin > 0.0625f) return;
   const int64_t numMC = t.levelOffset[t.numLevels];
   const float *S = m.gridletScalars.data();
   auto valid = [&](const Gridlet &g, int x, int y, int z) {
@@ -73,6 +71,17 @@ static void buildOwnerTables(const HostModel &m, LocateTables &t) {
   }
 }
 
+void gridletMargins(const HostModel &m, float &bandMargin, float &fastMargin) {
+  // the largest coordinate magnitude: cell bounds cover the gridlet boxes and
+  // the element vertices
+  float maxAbs = 1.f;
+  for (int k = 0; k < 6; ++k) maxAbs = std::max(maxAbs, fabsf(m.cellBounds[k]));
+  int e = 1;
+  while (e < 120 && ldexpf(1.f, e) <= maxAbs) ++e;
+  bandMargin = ldexpf(1.f, e - 21);
+  fastMargin = ldexpf(1.f, e - 20);
+}
+
 void buildLocateTables(const HostModel &m, LocateTables &t) {
   t = LocateTables();
   // ---- gridlets: per-level macrocell lists --------------------------------
@@ -111,6 +120,7 @@ void buildLocateTables(const HostModel &m, LocateTables &t) {
     for (size_t g = 0; g < m.gridlets.size(); ++g)
       forEachMC(m.gridlets[g], [&](int64_t i) { t.gList[cur[i]++] = (int32_t)g; });
   }
+  gridletMargins(m, t.bandMargin, t.fastMargin);
   buildOwnerTables(m, t);
   // ---- elements: uniform bins -----------------------------------------------
   const int64_t ne = m.numElements();
