@@ -193,6 +193,7 @@ constexpr int kFastLevels = 8;
 EXA_DEV Sample lookupTablesGridlets(const SamplerView &sv, const Culling &cu, const vec3f p, int32_t cached,
                                     bool &cachedHit) {
   cachedHit = false;
+  if (sv.numGridlets <= 0) return Sample{-1, -1, 0.f};  // no gridlet record to read
   const bool fast = sv.gTop != nullptr && sv.numLevels <= kFastLevels;
   // round 1: cached gridlet record, owner blocks of every level
   const Gridlet cg = sv.gridlets[cached >= 0 ? cached : 0];
