@@ -153,44 +153,58 @@ EXA_DEV void trackingRun(PathLane &L, const RenderParams &P, const GridView &gv,
   }
 }
 
-// end of a segment: the rest of the integrator step that traced it
+// end of a segment: the rest of the integrator step that traced it.  The
+// branches only decide; the pixel sample is finished, the light ray drawn and
+// (by the caller) the next segment begun at one place each, so a wave runs
+// that code once whatever mix of cases its lanes are in.
 template <bool kCount>
-EXA_DEV void segmentEnd(PathLane &L, const RenderParams &P, const GridView &gv, int numLights,
-                        LocalCounters &cnt) {
+EXA_DEV void segmentEnd(PathLane &L, const RenderParams &P, int numLights, LocalCounters &cnt, Ray &segRay,
+                        int &segKind) {
   const vec3f wpos = xfmPoint(P.voxelXfmInv, L.pos);
+  bool finish = false, light = false;
+  float4 res = make_float4(0.f, 0.f, 0.f, 0.f);
   if (L.kind == kSegPt) {
     if (L.itype == kBoundary || !ptInteraction(P, L.xf, L.throughput, L.bounce, L.rng)) {
-      endSample<kCount>(L, P, ptResult(L.Ld, L.throughput, L.bounce, numLights, backgroundColor(L.py, P.H)), cnt);
-      return;
-    }
-    L.spos = wpos;
-    if (numLights > 0) {
-      const Ray r = lightRay(P, wpos, numLights, L.rng, L.which, L.ld);
-      beginSegment(L, P, gv, r, kSegPtShadow);
+      res = ptResult(L.Ld, L.throughput, L.bounce, numLights, backgroundColor(L.py, P.H));
+      finish = true;
     } else {
-      L.st = kStScatter;
+      L.spos = wpos;
+      if (numLights > 0) {
+        light = true;
+        segKind = kSegPtShadow;
+      } else {
+        L.st = kStScatter;
+      }
     }
   } else if (L.kind == kSegPtShadow) {
     L.Ld = L.Ld + L.throughput * lightResult(P, L.which, L.Tr, L.ld);
     L.st = kStScatter;
   } else if (L.kind == kSegDl) {
     if (L.itype == kBoundary) {
-      endSample<kCount>(L, P, backgroundColor(L.py, P.H), cnt);
-      return;
-    }
-    const vec3f albedo = mk3(L.xf.x, L.xf.y, L.xf.z);
-    if (numLights > 0) {
-      L.Ld = albedo;
-      const Ray r = lightRay(P, wpos, numLights, L.rng, L.which, L.ld);
-      beginSegment(L, P, gv, r, kSegDlShadow);
+      res = backgroundColor(L.py, P.H);
+      finish = true;
     } else {
-      const vec3f th = mk3s(1.f) * albedo;
-      endSample<kCount>(L, P, make_float4(th.x, th.y, th.z, 1.f), cnt);
+      const vec3f albedo = mk3(L.xf.x, L.xf.y, L.xf.z);
+      if (numLights > 0) {
+        L.Ld = albedo;
+        light = true;
+        segKind = kSegDlShadow;
+      } else {
+        const vec3f th = mk3s(1.f) * albedo;
+        res = make_float4(th.x, th.y, th.z, 1.f);
+        finish = true;
+      }
     }
   } else {
     const vec3f th = mk3s(1.f) * (L.Ld * lightResult(P, L.which, L.Tr, L.ld));
-    endSample<kCount>(L, P, make_float4(th.x, th.y, th.z, 1.f), cnt);
+    res = make_float4(th.x, th.y, th.z, 1.f);
+    finish = true;
   }
+  if (light) {
+    segRay = lightRay(P, wpos, numLights, L.rng, L.which, L.ld);
+    L.st = kStCellBegin;  // the caller begins the shadow segment
+  }
+  if (finish) endSample<kCount>(L, P, res, cnt);
 }
 
 // pathTracingIntegrate after sampleLight: new direction, new path ray
@@ -207,33 +221,30 @@ EXA_DEV void scatter(PathLane &L, const RenderParams &P) {
 
 // camera ray and the integrators' entry (deviceCode.cu:1000-1027, 1130-1150)
 template <bool kCount>
-EXA_DEV void sampleStart(PathLane &L, const RenderParams &P, const GridView &gv, int numLights,
-                         LocalCounters &cnt) {
+EXA_DEV void sampleStart(PathLane &L, const RenderParams &P, int numLights, LocalCounters &cnt, Ray &segRay,
+                         int &segKind) {
   Ray ray = primaryRay(P, L.px, L.py, L.rng);
   float t0 = 1e30f, t1 = -1e30f;
-  if (P.integrator == 1) {
-    if (!boxTest(ray, P.worldBounds, t0, t1)) {
-      endSample<kCount>(L, P, backgroundColor(L.py, P.H), cnt);
-      return;
-    }
-    clipInterval(P, ray, t0, t1);
-    ray.org = ray.org + ray.dir * t0;
-    t1 -= t0;
-    ray.tmax = t1;
-    beginSegment(L, P, gv, ray, kSegDl);
-  } else {
+  const bool dl = P.integrator == 1;
+  if (!dl) {
     L.throughput = mk3s(1.f);
     L.Ld = mk3s(0.f);
     L.bounce = 0;
-    if (!boxTest(ray, P.worldBounds, t0, t1)) {
-      endSample<kCount>(L, P, ptResult(L.Ld, L.throughput, L.bounce, numLights, backgroundColor(L.py, P.H)),
-                        cnt);
-      return;
-    }
-    clipInterval(P, ray, t0, t1);
-    ray.org = ray.org + ray.dir * t0;
-    t1 -= t0;
-    ray.tmax = t1;
+  }
+  if (!boxTest(ray, P.worldBounds, t0, t1)) {
+    const float4 bg = backgroundColor(L.py, P.H);
+    endSample<kCount>(L, P, dl ? bg : ptResult(L.Ld, L.throughput, L.bounce, numLights, bg), cnt);
+    return;
+  }
+  clipInterval(P, ray, t0, t1);
+  ray.org = ray.org + ray.dir * t0;
+  t1 -= t0;
+  ray.tmax = t1;
+  if (dl) {
+    segRay = ray;
+    segKind = kSegDl;
+    L.st = kStCellBegin;  // the caller begins the segment
+  } else {
     L.ray = ray;
     L.st = kStPtSegment;
   }
@@ -322,18 +333,26 @@ kRenderPersistent(RenderParams P, SamplerView sv, GridView gv, PersistParams pp,
     // rare pass, in the order a lane flows through it
     const uint64_t rare = __ballot(isRare(L.st));
     if (rare && (stuck || __popcll(rare) * 64 >= nLive * pp.rareBatch)) {
+      // segRay / segKind: the segment a lane begins at the end of the pass
+      Ray segRay;
+      int segKind = -1;
       {
         const BatchRequest *R = kBatch ? pp.reqs + L.req : nullptr;
         const RenderParams &Pr = kBatch ? R->P : P;
-        if (L.st == kStSegEnd) segmentEnd<kCount>(L, Pr, gv, kBatch ? R->numLights : numLights1, cnt);
+        if (L.st == kStSegEnd) segmentEnd<kCount>(L, Pr, kBatch ? R->numLights : numLights1, cnt, segRay, segKind);
         if (L.st == kStScatter) scatter(L, Pr);
       }
       grabPixels<kBatch>(L, P, pp, workBegin, lane);
-      // a grab may have moved the lane to another request
+      // a grab may have moved the lane to another request (a lane that
+      // begins a shadow segment does not grab)
       const BatchRequest *R = kBatch ? pp.reqs + L.req : nullptr;
       const RenderParams &Pr = kBatch ? R->P : P;
-      if (L.st == kStSampleStart) sampleStart<kCount>(L, Pr, gv, kBatch ? R->numLights : numLights1, cnt);
-      if (L.st == kStPtSegment) beginSegment(L, Pr, gv, L.ray, kSegPt);
+      if (L.st == kStSampleStart) sampleStart<kCount>(L, Pr, kBatch ? R->numLights : numLights1, cnt, segRay, segKind);
+      if (L.st == kStPtSegment) {
+        segRay = L.ray;
+        segKind = kSegPt;
+      }
+      if (segKind >= 0) beginSegment(L, Pr, gv, segRay, segKind);
     }
 
     // this lane's request (after any grab): parameters, majorants and
