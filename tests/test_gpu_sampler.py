@@ -188,3 +188,52 @@ def test_locate_noise_scene(noise_scene, device, accel):
     kinds = set(np.unique(np.count_nonzero(sc.idx[ref[ref[:, 1] < 0][:, 0][ref[ref[:, 1] < 0][:, 0] >= 0]] >= 0,
                                            axis=1)))
     assert {5, 6, 8} <= kinds
+
+
+@pytest.mark.parametrize("accel", [ACCEL_TABLES, ACCEL_BVH], ids=["tables", "bvh"])
+def test_elements_only_model(cfg1, device, accel):
+    """A model with stitching elements and no gridlets (the gridlet stage has
+    nothing to read): point location against brute force, and a rendered
+    frame of both render schemes against the oracle."""
+    import torch
+    from owlexastitcher_amd import ExaStitchModel, OWLRenderer, RENDER_MEGAKERNEL, RENDER_PERSISTENT, camera_look_at
+    print(f"case: elements-only model, accel {accel}", flush=True)
+    model = ExaStitchModel.from_arrays(cfg1.verts, cfg1.idx, np.zeros((0, 8), np.int32), np.zeros(0, np.float32))
+    model.setNumGridCells((16, 16, 16))
+    s = ExaStitchSampler()
+    s.setAccel(accel)
+    s.build(model)
+    rng = np.random.default_rng(5)
+    cb, _ = model.bounds()
+    pos = rng.uniform(cb[:3], cb[3:], (20000, 3)).astype(np.float32)
+    ids, vals = _locate_gpu(s, pos, device)
+    om = ora.Model(cfg1.verts, cfg1.idx, np.zeros((0, 8), np.int32), np.zeros(0, np.float32))
+    ref, _ = om.locate_brute(pos)
+    mism, rel, _ = _compare(ids, vals, ref)
+    assert mism == 0 and rel <= VALUE_RTOL
+    assert np.count_nonzero(ids[:, 0] >= 0) > 100 and np.all(ids[ids[:, 0] >= 0, 1] == -1)
+    W, H = 40, 30
+    accs = []
+    for scheme in (RENDER_PERSISTENT, RENDER_MEGAKERNEL):
+        r = OWLRenderer(model, (16, 16, 16))
+        r.setAccel(accel)
+        r.setScheme(scheme)
+        b = r.modelBounds
+        c = 0.5 * (b[:3] + b[3:])
+        r.resize(W, H)
+        r.setCamera(*camera_look_at(c + np.array([-0.3, 0.7, 1.0]) * (b[3:] - b[:3]), c, (0, 1, 0), 60.0, W, H))
+        rgba = scenes.tf128()
+        r.setColorMap(rgba)
+        vr = r.valueRange
+        r.setRange(vr[0], vr[1])
+        r.setOpacityScale(120.0)
+        r.setMaxBounces(4)
+        st = r.state()
+        fb = torch.zeros((H, W), dtype=torch.int32, device=device)
+        r.render(fb)
+        torch.cuda.synchronize()
+        accs.append(r.accum())
+    maj = r.sampler.max_opacities()
+    om.set_culling(None, om.element_max_opacity(rgba, np.array(st.tfDomain, np.float32)))
+    acc_o, _ = ora.render_f32(ora.render_params_from_state(st, maj, rgba), om)
+    assert np.array_equal(accs[0], acc_o) and np.array_equal(accs[1], acc_o)
