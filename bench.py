@@ -517,6 +517,77 @@ def locate_workload(r, args, dev):
     }
 
 
+def bricks_workload(amr, r, args, dev):
+    """The ExaBricks comparison sampler (sampler/ExaBrickSampler.h, ABR mode)
+    on the same AMR leaf cells, view, transfer function, light and bounce cap
+    as the 1080p frame: bricks from the brick maker (16^3-cell blocks),
+    regions with per-region majorants walked on their kd-tree, every sample
+    a blend over the region's bricks.  The exaStitch frame of the same run is
+    timed beside it.  Algorithmic bytes: 64 B per brick blended (32 B record
+    + 8 corner scalars), 36 B per region entered (box + majorant), 36 B per
+    pixel."""
+    import torch
+    from owlexastitcher_amd import ExaBrickModel, OWLRenderer
+    t = time.time()
+    bm = ExaBrickModel.from_amr(amr, 16)
+    bc = bm.counts()
+    log(f"[bench] brick model: {bc} ({time.time() - t:.1f}s)")
+    t = time.time()
+    rb = OWLRenderer(bm, (args.mcs, args.mcs, args.mcs))
+    log(f"[bench] brick renderer: {time.time() - t:.1f}s")
+    st = r.state()
+    rb.resize(args.width, args.height)
+    rb.setCamera(st.camOrg, st.camDir00, st.camDu, st.camDv)
+    from owlexastitcher_amd import tf_rasterize
+    rb.setColorMap(tf_rasterize(TF5, 128))
+    vr = rb.valueRange
+    rb.setRange(vr[0], vr[1])
+    rb.setRelDomain(0.0, 100.0)
+    rb.setOpacityScale(100.0)
+    rb.setType(args.integrator)
+    rb.setMaxBounces(args.bounces)
+    rb.setSpp(args.spp)
+    rb.setDirectionalLight(0, (0.3, 1.0, 0.5), 2.0, True)
+    stream = torch.cuda.current_stream()
+    fb = torch.zeros((args.height, args.width), dtype=torch.int32, device=dev)
+
+    def timed(renderer):
+        for _ in range(args.warmup):
+            renderer.render(fb, stream)
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ev[0].record(stream)
+        for _ in range(args.steps):
+            renderer.render(fb, stream)
+        ev[1].record(stream)
+        torch.cuda.synchronize()
+        return ev[0].elapsed_time(ev[1]) / args.steps, (time.perf_counter() - t0) / args.steps * 1e3
+
+    ms_b, wall_b = timed(rb)
+    ms_s, wall_s = timed(r)
+    cnt = rb.render_counted(fb, stream)
+    alg = 64 * cnt["bricks"] + 36 * cnt["regions"] + 36 * cnt["pixels"]
+    achieved = alg / (ms_b * 1e-3) / 1e9
+    spf = args.width * args.height * args.spp
+    log(f"[bench] bricks: {ms_b:.2f} ms/frame ({spf / (ms_b * 1e-3) / 1e6:.0f}M samples/s), exaStitch "
+        f"{ms_s:.2f} ms/frame; counters {cnt}; {achieved:.0f} GB/s algorithmic")
+    return {
+        "metric": METRIC, "value": spf / (wall_b * 1e-3), "unit": "samples/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": wall_b, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": f"ExaBricks comparison sampler on the same {len(amr)} leaf cells "
+                               f"({bc['bricks']} bricks, {bc['abrs']} regions), {args.width}x{args.height}, "
+                               f"{args.spp} spp/frame, path tracer <= {args.bounces} bounces, 1 directional light",
+                   "brick_model": bc, "frame_ms": ms_b, "exastitch_frame_ms": ms_s,
+                   "exastitch_samples_per_s": spf / (wall_s * 1e-3),
+                   "volume_samples_per_s": cnt["samples"] / (ms_b * 1e-3), "counters": cnt},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "kRenderBricks"},
+        "cpu_baseline": None,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -538,10 +609,11 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--locate-points", type=int, default=1 << 24)
-    ap.add_argument("--workload", choices=["frame", "batch", "neural", "large", "locate"], default="frame",
+    ap.add_argument("--workload", choices=["frame", "batch", "neural", "large", "locate", "bricks"], default="frame",
                     help="frame: the 1080p frame (default); batch: batched requests in one launch; "
                          "neural: exact vs neural-field element stage; large: the 400M-cell shock scene, "
-                         "batched 4K + 16 side views with and without the network")
+                         "batched 4K + 16 side views with and without the network; locate: the sampler alone; "
+                         "bricks: the ExaBricks comparison sampler beside the exaStitch frame")
     ap.add_argument("--large-scale", type=int, default=1, help="large workload: base resolution and radius / k")
     ap.add_argument("--large-spp", type=int, default=64)
     ap.add_argument("--large-nmax", type=int, default=2048, help="large workload: finest hash-grid resolution")
@@ -581,6 +653,9 @@ def main():
         return
     if args.workload == "locate":
         print(json.dumps(locate_workload(r, args, dev)), flush=True)
+        return
+    if args.workload == "bricks":
+        print(json.dumps(bricks_workload(amr, r, args, dev)), flush=True)
         return
     stream = torch.cuda.current_stream()
     fb = torch.zeros((args.height, args.width), dtype=torch.int32, device=dev)

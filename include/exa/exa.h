@@ -382,6 +382,87 @@ int exa_neural_get_flags(const exa_neural *n, uint8_t *out); /* 1 per macrocell 
    table accelerator) */
 int exa_renderer_set_neural(exa_renderer *r, exa_neural *n);
 
+/* ---- ExaBricks comparison sampler (model/ExaBrickModel.h:27-56,
+        model/ABRs.h:28-100, sampler/ExaBrickSampler.h:30-241) ----
+   The reference's basis-function sampler over overlapping bricks of AMR
+   cells, in its default configuration (EXABRICK_ABR_TRAVERSAL with the ABR
+   sampling BVH, CMakeLists.txt:189-195): active brick regions (ABRs, boxes
+   in which one set of bricks overlaps) carry the majorants; a sample is the
+   blend of the tent basis functions of the region's bricks.  The regions are
+   found on the kd-tree that their build's recursion describes (in place of
+   the OptiX BVH over them). */
+/* model/ABRs.h:28-63 ExaBrick (32 bytes): size[0]*size[1]*size[2] cells of
+   width 2^level from lower (finest-cell units); cell (x,y,z) has scalar
+   scalars[begin + x + size[0]*(y + size[1]*z)] */
+typedef struct {
+  int32_t lower[3];
+  int32_t size[3];
+  int32_t level;
+  uint32_t begin;
+} exa_brick;
+/* model/ABRs.h:65-75 ABR (44 bytes) */
+typedef struct {
+  float domain[6]; /* lower xyz, upper xyz */
+  float valueRange[2];
+  int32_t leafListBegin;
+  int32_t leafListSize;
+  float finestLevelCellWidth;
+} exa_abr;
+typedef struct exa_brick_model exa_brick_model;
+/* ExaBrickModel::load(bricks, scalars, numBricks) (ExaBrickModel.cpp:119-145)
+   and init (148-213): copies; numCells = the last brick's begin + cells */
+int exa_brick_model_create(const exa_brick *bricks, const float *scalars, size_t numBricks, exa_brick_model **out);
+/* ExaBrickModel::load(brickFile, scalarFile, "") (ExaBrickModel.cpp:27-117):
+   .bricks = per brick size (3 int32), lower (3 int32), level (int32), then
+   its cell IDs (int32, x fastest) into the .scalars file */
+int exa_brick_model_load(const char *bricksPath, const char *scalarsPath, exa_brick_model **out);
+/* Brick maker (no reference counterpart; the reference reads bricks made by
+   external tools): each level's leaf cells, in blocks of blockCells^3 cells
+   (1..16), covered greedily by dense boxes; cell IDs = input order */
+int exa_brick_model_from_amr(const exa_amr *a, int blockCells, exa_brick_model **out);
+int exa_amr_write_bricks(const exa_amr *a, int blockCells, const char *bricksPath, const char *scalarsPath);
+/* counts[6]: bricks, brick cells, ABRs, leaf-list entries, kd nodes, kd depth */
+int exa_brick_model_get_counts(const exa_brick_model *m, int64_t counts[6]);
+/* copies, sized by exa_brick_model_get_counts; NULL skips one */
+int exa_brick_model_get(const exa_brick_model *m, exa_brick *bricks, float *scalars, exa_abr *abrs,
+                        int32_t *leafList);
+/* kd nodes, 2 int32 each: split (float bits), info = (lower child << 2) | dim
+   for a plane (upper child = next node), ((ABR + 1) << 2) | 3 for a leaf */
+int exa_brick_model_get_kd_nodes(const exa_brick_model *m, int32_t *nodes);
+/* cellBounds = union of brick bounds, valueRange = union of ABR ranges
+   (ExaBrickModel.cpp:177-187) */
+int exa_brick_model_get_bounds(const exa_brick_model *m, float cellBounds[6], float valueRange[2]);
+void exa_brick_model_destroy(exa_brick_model *m);
+
+typedef struct exa_brick_sampler exa_brick_sampler;
+/* ExaBrickSampler::build in ABR mode (ExaBrickSampler.cpp:26-247) */
+int exa_brick_sampler_create(const exa_brick_model *m, void *stream, exa_brick_sampler **out);
+/* computeMaxOpacitiesForABRs (ExaBrickSampler.cu:65-98, 100-118); a region
+   with max opacity 0 is culled (ExaBrickABRGeomBounds,
+   sampler/programs/ExaBrickSampler.cu:30-43) */
+int exa_brick_sampler_compute_max_opacities(exa_brick_sampler *s, const float *rgba, int numColors, float lo,
+                                            float hi, void *stream);
+int exa_brick_sampler_get_max_opacities(const exa_brick_sampler *s, float *hostOut); /* 1 per ABR */
+/* sample() by point (ExaBrickSampler.h:221-236, ExaBrickABRGeomSamplingIsect
+   sampler/programs/ExaBrickSampler.cu:69-94): {0, -1, value} where a region
+   that is not culled contains the point, else {0, 0, 0} */
+int exa_brick_sampler_locate(const exa_brick_sampler *s, const float *d_pos, int64_t n, exa_sample *d_out,
+                             void *stream);
+void exa_brick_sampler_destroy(exa_brick_sampler *s);
+/* OWLRenderer(const ExaBrick *, const float *, size_t, remap_from, remap_to,
+   numMCs) (OWLRenderer.cpp:269-291) from a brick model (copied); the
+   exa_renderer_* setters, render, render_counted, get_accum and get_state
+   apply.  renderFrame_ExaBrickSampler shades with Default only (shade mode
+   ignored); the ray marcher is the ExaBricks integrateDVR (deviceCode.cu:
+   401-499).  Render counters: [0] samples blended, [1] brick records read,
+   [2] kd leaves visited, [7] regions entered, [8] pixels, [9] TF fetches.
+   Accelerator, scheme, batch and neural entry points return EXA_ERR_STATE
+   (scheme and tuning are ignored). */
+int exa_renderer_create_bricks(const exa_brick_model *m, const int32_t numMCs[3], const float remapFrom[6],
+                               const float remapTo[6], exa_renderer **out);
+/* the renderer's brick sampler, or NULL for an exaStitch renderer */
+const exa_brick_sampler *exa_renderer_get_brick_sampler(const exa_renderer *r);
+
 /* ---- host helpers ---- */
 /* Camera from eye/point-of-interest/up/fovy for a W x H image
    (viewer.cpp:79-107 SimpleCamera, 550-560 cameraChanged). */

@@ -34,6 +34,17 @@
  *   dual_build                                   : cell-count invariants,
  *       perfect cubes == gridlet cells, element volumes tile the
  *       stitching region (sum of volumes), analytic cases (one-level box).
+ *   expf / powf                                  : error bound vs libm.
+ *   abrs_build (ExaBricks regions)               : the regions tile the union
+ *       of brick domains without overlap, each leaf list is exactly the set
+ *       of bricks overlapping the region, no brick face cuts a region, value
+ *       ranges recomputed from their definition, split-order invariants.
+ *   brick blend / locate                         : the reference's 2-brick
+ *       test data (tools/makeTestData.cpp) at known points, constant and
+ *       affine fields inside single bricks, partition of unity of the tent
+ *       weights, region blend == brute-force blend over all bricks.
+ *   brick rendering (region walk)                : closest-hit regions by
+ *       brute force over all regions; homogeneous closed forms.
  */
 #ifndef EXA_ORACLE_H
 #define EXA_ORACLE_H
@@ -128,6 +139,8 @@ uint32_t ora_lcg_init(uint32_t v0, uint32_t v1);
 float    ora_lcg_next(uint32_t *state);
 float    ora_logf(float x);
 void     ora_sincosf(float x, float *s, float *c);
+float    ora_expf(float x);
+float    ora_powf(float x, float y);
 
 /* ---- renderer ----------------------------------------------------------- */
 #define ORA_LIGHTS_MAX 4
@@ -161,6 +174,9 @@ typedef struct {
   int32_t debugCrosshair;  /* deviceCode.cu:1334-1342 (DEBUGGING == 1) */
   int32_t shadeMode;       /* deviceCode.cu:619-623: 0 default, 1 gridlets, 2 teaser */
   float   dt;              /* ray-marcher step (LaunchParams render.dt) */
+  /* ExaBricks scene (renderFrame_ExaBrickSampler, deviceCode.cu:1378-1388);
+     NULL: the exaStitch model passed to the render call */
+  const struct ora_brick_scene *bricks;
 } ora_render_params;
 
 /* Render rows [y0,y1) of one frame.  accum is float4[W*H] (in/out),
@@ -186,6 +202,47 @@ void ora_dual_free(ora_dual *d);
 
 /* model/ExaStitchModel.cpp:78-99,224-245: cellBounds and valueRange */
 void ora_model_bounds(const ora_model *m, float cellBounds[6], float valueRange[2]);
+
+/* ---- ExaBricks (model/ABRs.h, model/ABRs.cpp, sampler/ExaBrickSampler.h) -- */
+/* model/ABRs.h:28-63 ExaBrick */
+typedef struct {
+  int32_t  lower[3];
+  int32_t  size[3];
+  int32_t  level;
+  uint32_t begin;
+} ora_brick;
+/* model/ABRs.h:65-75 ABR */
+typedef struct {
+  float   domain[6];
+  float   valueRange[2];
+  int32_t leafListBegin;
+  int32_t leafListSize;
+  float   finestLevelCellWidth;
+} ora_abr;
+typedef struct ora_brick_scene {
+  const ora_brick *bricks;
+  int64_t          numBricks;
+  const float     *scalars;
+  const ora_abr   *abrs;
+  int64_t          numAbrs;
+  const int32_t   *leafList;
+  const float     *abrMaxOpacity; /* majorants; 0 = culled; NULL = none culled */
+} ora_brick_scene;
+
+/* ABRs::buildFrom (ABRs.cpp:272-352): regions in the reference's order */
+typedef struct ora_abrs ora_abrs;
+ora_abrs *ora_abrs_build(const ora_brick *bricks, int64_t n, const float *scalars);
+int64_t   ora_abrs_count(const ora_abrs *a, int what); /* 0 regions, 1 leaf-list entries */
+void      ora_abrs_get(const ora_abrs *a, ora_abr *abrs, int32_t *leafList);
+void      ora_abrs_free(ora_abrs *a);
+/* addBasisFunctions (ExaBrickSampler.h:102-202) summed over every brick
+   whose closed domain contains the point, ascending IDs: value and weight */
+void ora_brick_blend_brute_f32(const ora_brick *bricks, int64_t n, const float *scalars,
+                               const float *pos, int64_t npts, float *value, float *sumWeights);
+/* sample() by point (ExaBrickSampler.h:221-236): the lowest-ID region not
+   culled that contains the point, blend over its bricks; {0,-1,v}, or
+   {0,0,0} without weight */
+void ora_brick_locate_f32(const ora_brick_scene *s, const float *pos, int64_t n, ora_sample *out);
 
 #ifdef __cplusplus
 }

@@ -11,6 +11,8 @@
 #define FLOOR floorf
 #define LOGF ora_logf
 #define SINCOS ora_sincosf
+#define POWF ora_powf
+#define CEIL ceilf
 #include "ora_core.inc"
 
 /* ===================================================================== */
@@ -85,6 +87,42 @@ void ora_sincosf(float x, float *sn, float *cs) {
   case 2: *sn = -sr; *cs = -cr; break;
   default: *sn = -cr; *cs = sr; break;
   }
+}
+
+/* Own exp / pow (the same algorithms run in the HIP kernels).
+   expf: Cody & Waite reduction x = n ln2 + r (ln2 split as in ora_logf),
+   Taylor polynomial of e^r to r^7 on |r| <= ln2/2, times 2^n; results below
+   the smallest normal float are 0.  powf: exp(y log x) for x > 0, with
+   pow(x, 0) = 1, pow(1, y) = 1, pow(0, y > 0) = 0. */
+float ora_expf(float x) {
+  if (!(x == x)) return x;
+  if (x < -87.3f) return 0.f;
+  if (x > 88.72f) return INFINITY;
+  const float n = rintf(x * 1.44269502162933349609375f);
+  const float r = (x - n * 0.693145751953125f) - n * 1.428606765330187045e-06f;
+  const float p =
+      1.0f + r * (1.0f + r * (0.5f + r * (0.16666667163372039794921875f +
+                                          r * (0.0416666679084300994873046875f +
+                                               r * (0.008333333767950534820556640625f +
+                                                    r * (0.001388888922519981861114501953125f +
+                                                         r * 0.0001984127011382952332496643066406f))))));
+  int ni = (int)n;
+  float q = p;
+  if (ni > 127) {
+    q = q * 2.f;
+    ni -= 1;
+  }
+  union { uint32_t u; float f; } sc;
+  sc.u = (uint32_t)(ni + 127) << 23;
+  return q * sc.f;
+}
+
+float ora_powf(float x, float y) {
+  if (y == 0.f) return 1.f;
+  if (x == 1.f) return 1.f;
+  if (x == 0.f) return y > 0.f ? 0.f : INFINITY;
+  if (!(x > 0.f)) return NAN;
+  return ora_expf(y * ora_logf(x));
 }
 
 /* ===================================================================== */
@@ -480,4 +518,55 @@ void ora_render_f32(const ora_render_params *p, const ora_model *m, const ora_bi
                   (make_8bit(acc.z * is) << 16) + (make_8bit(acc.w * is) << 24);
       }
     }
+}
+
+/* ===================================================================== */
+/* ExaBricks sampling                                                      */
+/* ===================================================================== */
+static int brick_domain_contains(const ora_brick *b, const float *p) {
+  const float cw = (float)(1 << b->level);
+  for (int k = 0; k < 3; ++k) {
+    const float lo = (float)b->lower[k] - 0.5f * cw;
+    const float hi = (float)b->lower[k] + ((float)b->size[k] + 0.5f) * cw;
+    if (!(p[k] >= lo && p[k] <= hi)) return 0;
+  }
+  return 1;
+}
+
+void ora_brick_blend_brute_f32(const ora_brick *bricks, int64_t n, const float *scalars, const float *pos,
+                               int64_t npts, float *value, float *sumWeights) {
+  ora_brick_scene bs;
+  memset(&bs, 0, sizeof(bs));
+  bs.bricks = bricks;
+  bs.numBricks = n;
+  bs.scalars = scalars;
+  for (int64_t i = 0; i < npts; ++i) {
+    const float *p = pos + 3 * i;
+    float wv = 0.f, w = 0.f;
+    for (int64_t b = 0; b < n; ++b)
+      if (brick_domain_contains(&bricks[b], p)) brick_add_basis(&bs, b, v3(p[0], p[1], p[2]), &wv, &w);
+    value[i] = wv / w;
+    sumWeights[i] = w;
+  }
+}
+
+void ora_brick_locate_f32(const ora_brick_scene *s, const float *pos, int64_t n, ora_sample *out) {
+  for (int64_t i = 0; i < n; ++i) {
+    const float *p = pos + 3 * i;
+    ora_sample r = {0, 0, 0.f};
+    for (int64_t a = 0; a < s->numAbrs; ++a) {
+      if (abr_culled(s, a)) continue;
+      const float *d = s->abrs[a].domain;
+      if (!(p[0] >= d[0] && p[1] >= d[1] && p[2] >= d[2] && p[0] <= d[3] && p[1] <= d[4] && p[2] <= d[5]))
+        continue;
+      float w;
+      const float v = abr_blend(s, a, v3(p[0], p[1], p[2]), &w);
+      if (w > 0.f) {
+        r.cellID = -1;
+        r.value = v;
+      }
+      break;
+    }
+    out[i] = r;
+  }
 }

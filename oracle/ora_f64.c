@@ -10,6 +10,8 @@
 #define FLOOR floor
 #define LOGF log
 #define SINCOS sincos
+#define POWF pow
+#define CEIL ceil
 #define _GNU_SOURCE
 #include "ora_core.inc"
 

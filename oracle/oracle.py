@@ -44,6 +44,15 @@ class OraRenderParams(C.Structure):
         ("clipEnabled", C.c_int32), ("clipN", C.c_float * 3), ("clipD", C.c_float),
         ("lights", OraLight * 4),
         ("debugCrosshair", C.c_int32), ("shadeMode", C.c_int32), ("dt", C.c_float),
+        ("bricks", C.c_void_p),
+    ]
+
+
+class OraBrickScene(C.Structure):
+    _fields_ = [
+        ("bricks", C.c_void_p), ("numBricks", C.c_int64), ("scalars", C.c_void_p),
+        ("abrs", C.c_void_p), ("numAbrs", C.c_int64), ("leafList", C.c_void_p),
+        ("abrMaxOpacity", C.c_void_p),
     ]
 
 
@@ -79,7 +88,20 @@ _SIGS = {
     "ora_dual_get": (None, [P, P, P, P, P]),
     "ora_dual_free": (None, [P]),
     "ora_model_bounds": (None, [P, P, P]),
+    "ora_expf": (C.c_float, [C.c_float]),
+    "ora_powf": (C.c_float, [C.c_float, C.c_float]),
+    "ora_abrs_build": (P, [P, C.c_int64, P]),
+    "ora_abrs_count": (C.c_int64, [P, C.c_int]),
+    "ora_abrs_get": (None, [P, P, P]),
+    "ora_abrs_free": (None, [P]),
+    "ora_brick_blend_brute_f32": (None, [P, C.c_int64, P, P, C.c_int64, P, P]),
+    "ora_brick_locate_f32": (None, [P, P, C.c_int64, P]),
 }
+
+# numpy layouts of the ExaBricks records (model/ABRs.h:28-75)
+BRICK_DTYPE = np.dtype([("lower", "<i4", 3), ("size", "<i4", 3), ("level", "<i4"), ("begin", "<u4")])
+ABR_DTYPE = np.dtype([("domain", "<f4", 6), ("valueRange", "<f4", 2), ("leafListBegin", "<i4"),
+                      ("leafListSize", "<i4"), ("finestLevelCellWidth", "<f4")])
 
 _lib = None
 
@@ -228,6 +250,97 @@ def tf_lookup(rgba, domain, v):
 def lcg_stream(v0, v1, n):
     st = np.array([lib().ora_lcg_init(v0, v1)], np.uint32)
     return np.array([lib().ora_lcg_next(ptr(st)) for _ in range(n)], np.float32)
+
+
+def expf(x):
+    return lib().ora_expf(float(x))
+
+
+def powf(x, y):
+    return lib().ora_powf(float(x), float(y))
+
+
+def abrs_build(bricks, scalars):
+    """ABRs::buildFrom (ABRs.cpp:272-352): (abrs [ABR_DTYPE], leafList)."""
+    bricks = np.ascontiguousarray(bricks, BRICK_DTYPE)
+    scalars = np.ascontiguousarray(scalars, np.float32)
+    a = lib().ora_abrs_build(ptr(bricks), len(bricks), ptr(scalars))
+    abrs = np.zeros(lib().ora_abrs_count(a, 0), ABR_DTYPE)
+    leaf = np.zeros(lib().ora_abrs_count(a, 1), np.int32)
+    lib().ora_abrs_get(a, ptr(abrs), ptr(leaf))
+    lib().ora_abrs_free(a)
+    return abrs, leaf
+
+
+def brick_blend_brute(bricks, scalars, pos):
+    """Blend over every brick whose closed domain contains each point:
+    (value, sum of weights)."""
+    bricks = np.ascontiguousarray(bricks, BRICK_DTYPE)
+    scalars = np.ascontiguousarray(scalars, np.float32)
+    pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+    v = np.zeros(len(pos), np.float32)
+    w = np.zeros(len(pos), np.float32)
+    lib().ora_brick_blend_brute_f32(ptr(bricks), len(bricks), ptr(scalars), ptr(pos), len(pos), ptr(v), ptr(w))
+    return v, w
+
+
+class BrickScene:
+    """Bricks, scalars, the oracle's own regions and per-region max
+    opacities behind an OraBrickScene struct."""
+
+    def __init__(self, bricks, scalars, abr_max_op=None):
+        self.bricks = np.ascontiguousarray(bricks, BRICK_DTYPE)
+        self.scalars = np.ascontiguousarray(scalars, np.float32)
+        self.abrs, self.leaf = abrs_build(self.bricks, self.scalars)
+        self.s = OraBrickScene(ptr(self.bricks).value, len(self.bricks), ptr(self.scalars).value,
+                               ptr(self.abrs).value, len(self.abrs), ptr(self.leaf).value, None)
+        self.set_max_opacity(abr_max_op)
+
+    def set_max_opacity(self, m):
+        self.mo = None if m is None else np.ascontiguousarray(m, np.float32)
+        self.s.abrMaxOpacity = ptr(self.mo).value if self.mo is not None else None
+
+    def value_ranges(self):
+        return np.ascontiguousarray(self.abrs["valueRange"], np.float32)
+
+    @property
+    def ref(self):
+        return C.byref(self.s)
+
+    def locate(self, pos):
+        pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+        out = np.zeros((len(pos), 3), np.int32)
+        lib().ora_brick_locate_f32(self.ref, ptr(pos), len(pos), ptr(out))
+        return out
+
+    def bounds(self):
+        """cellBounds = union of brick bounds, valueRange = union of region
+        ranges (ExaBrickModel.cpp:177-187)"""
+        b = self.bricks
+        w = (1 << b["level"].astype(np.int64))[:, None]
+        lo = b["lower"].astype(np.int64)
+        hi = lo + b["size"].astype(np.int64) * w
+        cb = np.concatenate([lo.min(0), hi.max(0)]).astype(np.float32)
+        vr = np.array([self.abrs["valueRange"][:, 0].min(), self.abrs["valueRange"][:, 1].max()], np.float32)
+        return cb, vr
+
+
+def render_bricks_f32(params, scene, accum=None, threads=8):
+    """One frame of the ExaBricks renderer (params.bricks is set here)."""
+    from concurrent.futures import ThreadPoolExecutor
+    W, H = params.W, params.H
+    if accum is None:
+        accum = np.zeros((H, W, 4), np.float32)
+    fb = np.zeros((H, W), np.uint32)
+    params.bricks = C.cast(C.pointer(scene.s), C.c_void_p).value
+    rows = max(1, H // (threads * 4))
+
+    def work(y0):
+        lib().ora_render_f32(C.byref(params), None, None, ptr(accum), ptr(fb), y0, min(H, y0 + rows))
+
+    with ThreadPoolExecutor(threads) as ex:
+        list(ex.map(work, range(0, H, rows)))
+    return accum, fb
 
 
 def render_params_from_state(st, majorants, tf_rgba):

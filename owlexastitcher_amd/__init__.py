@@ -7,7 +7,8 @@ raises ImportError if it has not been built.
 """
 from ._capi import LIB_PATH, ExaError, lib  # noqa: F401
 from .host import (ACCEL_BVH, ACCEL_TABLES, DIRECT_LIGHTING, PATH_TRACER, RAY_MARCHER, RENDER_MEGAKERNEL,  # noqa: F401
-                   RENDER_PERSISTENT, AmrScene, DualMesh, ExaStitchModel, ExaStitchSampler, NeuralField, OWLRenderer,
+                   RENDER_PERSISTENT, AmrScene, DualMesh, ExaBrickModel, ExaBrickSampler, ExaStitchModel,
+                   ExaStitchSampler, NeuralField, OWLRenderer,
                    RenderBatch,
                    camera_look_at, make_grids,
                    tf_rasterize, xf_load, xf_save)

@@ -191,6 +191,22 @@ _SIGS = {
     "exa_camera_look_at": (C.c_int, [P, P, P, F, C.c_int, C.c_int, P, P, P, P]),
     "exa_xf_load": (C.c_int, [C.c_char_p, P, P, P, P, P, C.c_int]),
     "exa_tf_rasterize": (C.c_int, [P, C.c_int, C.c_int, P]),
+    "exa_brick_model_create": (C.c_int, [P, P, C.c_size_t, P]),
+    "exa_brick_model_load": (C.c_int, [C.c_char_p, C.c_char_p, P]),
+    "exa_brick_model_from_amr": (C.c_int, [P, C.c_int, P]),
+    "exa_amr_write_bricks": (C.c_int, [P, C.c_int, C.c_char_p, C.c_char_p]),
+    "exa_brick_model_get_counts": (C.c_int, [P, P]),
+    "exa_brick_model_get": (C.c_int, [P, P, P, P, P]),
+    "exa_brick_model_get_kd_nodes": (C.c_int, [P, P]),
+    "exa_brick_model_get_bounds": (C.c_int, [P, P, P]),
+    "exa_brick_model_destroy": (None, [P]),
+    "exa_brick_sampler_create": (C.c_int, [P, P, P]),
+    "exa_brick_sampler_compute_max_opacities": (C.c_int, [P, P, C.c_int, F, F, P]),
+    "exa_brick_sampler_get_max_opacities": (C.c_int, [P, P]),
+    "exa_brick_sampler_locate": (C.c_int, [P, P, I64, P, P]),
+    "exa_brick_sampler_destroy": (None, [P]),
+    "exa_renderer_create_bricks": (C.c_int, [P, P, P, P, P]),
+    "exa_renderer_get_brick_sampler": (P, [P]),
 }
 
 EXPORTED = sorted(_SIGS)

@@ -14,6 +14,8 @@
 #include "exa_kernels.h"
 #include "exa_io.h"
 #include "neural_kernels.h"
+#include "bricks.h"
+#include "bricks_kernels.h"
 
 namespace exa {
 
@@ -341,6 +343,94 @@ struct exa_sampler {
   }
 };
 
+// ExaBrickModel (model/ExaBrickModel.h:27-56) and ExaBrickSampler in its
+// ABR mode (sampler/ExaBrickSampler.cpp:26-247): bricks, scalars, regions
+// and their kd-tree on the device; per-region value ranges and max
+// opacities (the majorants, and culling where 0).
+struct exa_brick_model {
+  BrickModel m;
+};
+
+struct exa_brick_sampler {
+  DeviceBuffer<Brick> bricks;
+  DeviceBuffer<float> scalars;
+  DeviceBuffer<float4> abrLo, abrHi;
+  DeviceBuffer<float> abrCellWidth;
+  DeviceBuffer<int32_t> leafList;
+  DeviceBuffer<KdNode> nodes;
+  DeviceBuffer<float2> abrRanges;
+  DeviceBuffer<float> maxOp;
+  DeviceBuffer<float4> colorMap;
+  DeviceBuffer<unsigned long long> counters;
+  int32_t numAbrs = 0;
+  bool haveMaxOp = false;
+
+  void build(const BrickModel &m, hipStream_t s) {
+    if (m.bricks.empty() || m.abrs.empty()) throw Status(EXA_ERR_INVALID, "brick model without bricks");
+    if (m.abrs.size() >= (1u << 29) || m.nodes.size() >= (1u << 29))
+      throw Status(EXA_ERR_INVALID, "too many regions for the kd-tree encoding");
+    numAbrs = (int32_t)m.abrs.size();
+    std::vector<float4> lo(numAbrs), hi(numAbrs);
+    std::vector<float> cw(numAbrs);
+    std::vector<float2> ranges(numAbrs);
+    for (int32_t i = 0; i < numAbrs; ++i) {
+      const Abr &a = m.abrs[i];
+      int32_t b = a.leafListBegin, c = a.leafListSize;
+      float fb, fc;
+      memcpy(&fb, &b, 4);
+      memcpy(&fc, &c, 4);
+      lo[i] = make_float4(a.domain[0], a.domain[1], a.domain[2], fb);
+      hi[i] = make_float4(a.domain[3], a.domain[4], a.domain[5], fc);
+      cw[i] = a.finestLevelCellWidth;
+      ranges[i] = make_float2(a.valueRange[0], a.valueRange[1]);
+    }
+    // every walk on the device ends: children come after their parent
+    const int64_t numNodes = (int64_t)m.nodes.size();
+    for (int64_t i = 0; i < numNodes; ++i) {
+      const int32_t info = m.nodes[i].info, ref = info >> 2;
+      const bool ok = (info & 3) == 3 ? (ref >= 0 && ref <= numAbrs) : (i + 1 < numNodes && ref > i + 1 && ref < numNodes);
+      if (!ok) throw Status(EXA_ERR_INVALID, "invalid kd-tree node");
+    }
+    bricks.upload(m.bricks.data(), m.bricks.size(), s);
+    scalars.upload(m.scalars.data(), m.scalars.size(), s);
+    abrLo.upload(lo.data(), lo.size(), s);
+    abrHi.upload(hi.data(), hi.size(), s);
+    abrCellWidth.upload(cw.data(), cw.size(), s);
+    leafList.upload(m.leafList.data(), m.leafList.size(), s);
+    nodes.upload(m.nodes.data(), m.nodes.size(), s);
+    abrRanges.upload(ranges.data(), ranges.size(), s);
+    maxOp.alloc(numAbrs);
+    counters.alloc(kNumCounters);
+    HIP_CHECK(hipStreamSynchronize(s));
+    haveMaxOp = false;
+  }
+
+  BrickView view() const {
+    BrickView v;
+    v.bricks = (const int4 *)bricks.ptr;
+    v.scalars = scalars.ptr;
+    v.abrLo = abrLo.ptr;
+    v.abrHi = abrHi.ptr;
+    v.abrCellWidth = abrCellWidth.ptr;
+    v.leafList = leafList.ptr;
+    v.nodes = (const int2 *)nodes.ptr;
+    v.abrMaxOp = haveMaxOp ? maxOp.ptr : nullptr;
+    v.numAbrs = numAbrs;
+    return v;
+  }
+
+  // ExaBrickSampler::computeMaxOpacities in ABR mode (ExaBrickSampler.cu:65-98,
+  // 110-118): computeMaxOpacitiesForABRs, the same mapping as the grid's
+  void computeMaxOpacities(const float *rgba, int n, float lo, float hi, hipStream_t s) {
+    require(rgba && n > 0, "empty color map");
+    colorMap.upload((const float4 *)rgba, n, s);
+    launchMapMaxOpacity(abrRanges.ptr, numAbrs, colorMap.ptr, n, lo, hi, maxOp.ptr, s);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipStreamSynchronize(s));
+    haveMaxOp = true;
+  }
+};
+
 // Neural field sampler state: parameters (float master, bf16 copies), Adam
 // moments, the training pool, and the per-macrocell validation result.
 struct exa_neural {
@@ -412,12 +502,18 @@ struct exa_renderer {
   exa_neural *neural = nullptr;  // neural field for validated macrocells (not owned)
   PersistTuning tuning;
   DeviceBuffer<uint32_t> work;
+  // set for a renderer over an ExaBricks model (then `sampler` stays unbuilt)
+  std::unique_ptr<exa_brick_sampler> bricks;
 
   void updateMaxOpacities() {
     if (colorMap.empty()) return;
     // OWLRenderer.cpp:647-652
     const float lo = absDomain[0] + (relDomain[0] / 100.f) * (absDomain[1] - absDomain[0]);
     const float hi = absDomain[0] + (relDomain[1] / 100.f) * (absDomain[1] - absDomain[0]);
+    if (bricks) {
+      bricks->computeMaxOpacities(colorMap.data(), (int)colorMap.size() / 4, lo, hi, 0);
+      return;
+    }
     sampler.computeMaxOpacities(colorMap.data(), (int)colorMap.size() / 4, lo, hi, 0);
     if (neural && neural->validated) {
       // majorants of the macrocells the network answers: the TF maximum over
@@ -922,6 +1018,56 @@ int exa_sampler_get_accel(const exa_sampler *s) { return s ? s->accel : EXA_ERR_
 void exa_sampler_destroy(exa_sampler *s) { delete s; }
 
 // ---- renderer --------------------------------------------------------------
+// entry points of the exaStitch sampler only
+static void requireStitch(const exa_renderer *r) {
+  if (r && r->bricks) throw Status(EXA_ERR_STATE, "not available for a renderer over an ExaBricks model");
+}
+
+// OWLRenderer constructors after the model load (OWLRenderer.cpp:166-181,
+// 278-291) and the parameter defaults of initGPU / render (309-622).
+static void initRenderer(exa_renderer *r, const float cellBounds[6], const float valueRange[2],
+                         const float remapFrom[6], const float remapTo[6], float voxelXfm[12]) {
+  float cellSize[3] = {cellBounds[3] - cellBounds[0], cellBounds[4] - cellBounds[1], cellBounds[5] - cellBounds[2]};
+  float lightScale = 1.f;
+  while (fmaxf(fmaxf(cellSize[0], cellSize[1]), cellSize[2]) > 1000.f) {
+    for (int k = 0; k < 3; ++k) cellSize[k] /= 1000.f;
+    lightScale /= 1000.f;
+  }
+  static const float unit[6] = {0, 0, 0, 1, 1, 1};
+  voxelTransform(remapFrom ? remapFrom : unit, remapTo ? remapTo : unit, voxelXfm);
+  RenderParams &P = r->P;
+  memset(&P, 0, sizeof(P));
+  memcpy(P.voxelXfm, voxelXfm, sizeof(P.voxelXfm));
+  affineInverse(voxelXfm, P.voxelXfmInv);
+  // Model::getBounds (Model.cpp:50-70)
+  xfmPointHost(P.voxelXfmInv, cellBounds, r->modelBounds);
+  xfmPointHost(P.voxelXfmInv, cellBounds + 3, r->modelBounds + 3);
+  memcpy(r->valueRange, valueRange, sizeof(r->valueRange));
+  affineIdentity(P.lightXfm);
+  P.lightXfm[0] = P.lightXfm[4] = P.lightXfm[8] = lightScale;
+  memcpy(P.worldBounds, r->modelBounds, sizeof(P.worldBounds));
+  P.clipEnabled = 0;
+  P.clipN[0] = 0.f;
+  P.clipN[1] = 0.f;
+  P.clipN[2] = 1.f;
+  P.clipD = 0.5f * (r->modelBounds[2] + r->modelBounds[5]);
+  P.subImage[0] = 0.f;
+  P.subImage[1] = 0.f;
+  P.subImage[2] = 0.f;
+  P.subImage[3] = 1.f;
+  P.subImageSel[2] = 1.f;
+  P.subImageSel[3] = 1.f;
+  P.integrator = 0;
+  P.shadeMode = 0;
+  P.maxBounces = 1024;
+  P.debugCrosshair = 1;
+  P.dt = 1.f;  // OWLRenderer.cpp:616 sets render.dt = 1 for every frame
+  P.heatMapEnabled = 0;
+  P.heatMapScale = 1.f;
+  P.opacityScale = 1.f;
+  P.spp = 1;
+}
+
 int exa_renderer_create(const exa_model *m, const int32_t numMCs[3], const float remapFrom[6],
                         const float remapTo[6], exa_renderer **out) {
   return guarded([&] {
@@ -929,50 +1075,28 @@ int exa_renderer_create(const exa_model *m, const int32_t numMCs[3], const float
     auto r = std::make_unique<exa_renderer>();
     r->model = m->m;
     HostModel &M = r->model;
-    // OWLRenderer.cpp:166-181
-    float cellSize[3] = {M.cellBounds[3] - M.cellBounds[0], M.cellBounds[4] - M.cellBounds[1],
-                         M.cellBounds[5] - M.cellBounds[2]};
-    float lightScale = 1.f;
-    while (fmaxf(fmaxf(cellSize[0], cellSize[1]), cellSize[2]) > 1000.f) {
-      for (int k = 0; k < 3; ++k) cellSize[k] /= 1000.f;
-      lightScale /= 1000.f;
-    }
-    static const float unit[6] = {0, 0, 0, 1, 1, 1};
-    voxelTransform(remapFrom ? remapFrom : unit, remapTo ? remapTo : unit, M.voxelXfm);
+    initRenderer(r.get(), M.cellBounds, M.valueRange, remapFrom, remapTo, M.voxelXfm);
     for (int k = 0; k < 3; ++k) M.numMCs[k] = numMCs[k];
-    RenderParams &P = r->P;
-    memset(&P, 0, sizeof(P));
-    memcpy(P.voxelXfm, M.voxelXfm, sizeof(P.voxelXfm));
-    affineInverse(M.voxelXfm, P.voxelXfmInv);
-    // Model::getBounds (Model.cpp:50-70)
-    xfmPointHost(P.voxelXfmInv, M.cellBounds, r->modelBounds);
-    xfmPointHost(P.voxelXfmInv, M.cellBounds + 3, r->modelBounds + 3);
-    memcpy(r->valueRange, M.valueRange, sizeof(r->valueRange));
-    affineIdentity(P.lightXfm);
-    P.lightXfm[0] = P.lightXfm[4] = P.lightXfm[8] = lightScale;
     // initGPU (OWLRenderer.cpp:309-582)
     r->sampler.build(M, 0);
-    memcpy(P.worldBounds, r->modelBounds, sizeof(P.worldBounds));
-    P.clipEnabled = 0;
-    P.clipN[0] = 0.f;
-    P.clipN[1] = 0.f;
-    P.clipN[2] = 1.f;
-    P.clipD = 0.5f * (r->modelBounds[2] + r->modelBounds[5]);
-    P.subImage[0] = 0.f;
-    P.subImage[1] = 0.f;
-    P.subImage[2] = 0.f;
-    P.subImage[3] = 1.f;
-    P.subImageSel[2] = 1.f;
-    P.subImageSel[3] = 1.f;
-    P.integrator = 0;
-    P.shadeMode = 0;
-    P.maxBounces = 1024;
-    P.debugCrosshair = 1;
-    P.dt = 1.f;  // OWLRenderer.cpp:616 sets render.dt = 1 for every frame
-    P.heatMapEnabled = 0;
-    P.heatMapScale = 1.f;
-    P.opacityScale = 1.f;
-    P.spp = 1;
+    *out = r.release();
+  });
+}
+// OWLRenderer(const ExaBrick *, const float *, size_t, ...) (OWLRenderer.cpp:
+// 269-291) with the ExaBrickSampler in ABR mode; numMCs is kept for the
+// signature (the ABR traversal uses no macrocell grid).
+int exa_renderer_create_bricks(const exa_brick_model *m, const int32_t numMCs[3], const float remapFrom[6],
+                               const float remapTo[6], exa_renderer **out) {
+  return guarded([&] {
+    require(m && out, "invalid arguments");
+    auto r = std::make_unique<exa_renderer>();
+    float voxelXfm[12];
+    initRenderer(r.get(), m->m.cellBounds, m->m.valueRange, remapFrom, remapTo, voxelXfm);
+    if (numMCs)
+      for (int k = 0; k < 3; ++k) r->model.numMCs[k] = numMCs[k];
+    memcpy(r->model.voxelXfm, voxelXfm, sizeof(voxelXfm));
+    r->bricks = std::make_unique<exa_brick_sampler>();
+    r->bricks->build(m->m, 0);
     *out = r.release();
   });
 }
@@ -1006,6 +1130,12 @@ static void doRender(exa_renderer *r, uint32_t *fb, unsigned long long *counters
   P.spp = r->spp < 1 ? 1 : r->spp;
   P.tf = r->tf.ptr;
   P.tfN = (int32_t)r->tf.count;
+  if (r->bricks) {
+    launchRenderBricks(P, r->bricks->view(), counters, s);
+    HIP_CHECK(hipGetLastError());
+    r->accumID++;
+    return;
+  }
   // the ray marcher and the heat map (a per-pixel clock) run in the
   // one-thread-per-pixel kernel
   const bool mega = r->scheme != EXA_RENDER_PERSISTENT || P.integrator == 2 || P.heatMapEnabled;
@@ -1036,7 +1166,7 @@ int exa_renderer_render_counted(exa_renderer *r, uint32_t *d_fb, uint64_t counte
   return guarded([&] {
     require(r && counters, "invalid arguments");
     hipStream_t st = (hipStream_t)stream;
-    unsigned long long *dc = r->sampler.counters.ptr;
+    unsigned long long *dc = r->bricks ? r->bricks->counters.ptr : r->sampler.counters.ptr;
     HIP_CHECK(hipMemsetAsync(dc, 0, kNumCounters * sizeof(unsigned long long), st));
     doRender(r, d_fb, dc, st);
     unsigned long long h[kNumCounters];
@@ -1211,6 +1341,7 @@ const exa_sampler *exa_renderer_get_sampler(const exa_renderer *r) { return r ? 
 int exa_renderer_set_accel(exa_renderer *r, int accel) {
   return guarded([&] {
     require(r, "renderer is NULL");
+    requireStitch(r);
     r->sampler.setAccel(accel, &r->model, 0);
   });
 }
@@ -1237,6 +1368,7 @@ int exa_renderer_set_persistent_tuning(exa_renderer *r, int ctrlSteps, int elemB
 int exa_batch_create(exa_renderer *r, exa_batch **out) {
   return guarded([&] {
     require(r && out, "invalid arguments");
+    requireStitch(r);
     auto b = std::make_unique<exa_batch>();
     b->r = r;
     *out = b.release();
@@ -1485,6 +1617,7 @@ int exa_neural_create(exa_renderer *r, int levels, int log2T, float nMin, float 
                       exa_neural **out) {
   return guarded([&] {
     require(r && out, "invalid arguments");
+    requireStitch(r);
     require(levels >= 1 && levels <= kNeuralMaxLevels, "levels must be in [1, 16]");
     require(log2T >= 4 && log2T <= 24, "log2T must be in [4, 24]");
     require(nMin >= 1.f && nMax >= nMin && nMax <= 65536.f, "need 1 <= nMin <= nMax <= 65536");
@@ -1587,6 +1720,7 @@ int exa_neural_make_pool(exa_neural *n, exa_renderer *r, int64_t count, float el
                          int64_t *numValid) {
   return guarded([&] {
     require(n && r && count > 0 && count < (1ll << 31), "invalid arguments");
+    requireStitch(r);
     exa_sampler &S = r->sampler;
     DeviceBuffer<float> pos;
     DeviceBuffer<Sample> smp;
@@ -1714,6 +1848,7 @@ int exa_neural_validate(exa_neural *n, exa_renderer *r, int pointsPerMC, float b
                         float out[3]) {
   return guarded([&] {
     require(n && r && pointsPerMC > 0 && bound >= 0.f, "invalid arguments");
+    requireStitch(r);
     exa_sampler &S = r->sampler;
     const GridView gv = S.gridView();
     const int64_t nmc = (int64_t)gv.dims[0] * gv.dims[1] * gv.dims[2];
@@ -1782,6 +1917,7 @@ int exa_neural_get_flags(const exa_neural *n, uint8_t *out) {
 int exa_renderer_set_neural(exa_renderer *r, exa_neural *n) {
   return guarded([&] {
     require(r, "renderer is NULL");
+    requireStitch(r);
     require(!n || n->validated, "the neural field must be validated first (exa_neural_validate)");
     r->neural = n;
     r->updateMaxOpacities();
@@ -1912,6 +2048,116 @@ int exa_tf_rasterize(const float *pts, int np, int N, float *rgba) {
       for (int c = 0; c < 4; ++c) rgba[4 * i + c] = a[1 + c] + t * (b[1 + c] - a[1 + c]);
     }
   });
+}
+
+// ---- ExaBricks comparison sampler -------------------------------------------
+int exa_brick_model_create(const exa_brick *bricks, const float *scalars, size_t numBricks, exa_brick_model **out) {
+  return guarded([&] {
+    require(bricks && scalars && numBricks > 0 && out, "invalid arguments");
+    auto m = std::make_unique<exa_brick_model>();
+    brickModelFromArrays((const Brick *)bricks, scalars, numBricks, m->m);
+    *out = m.release();
+  });
+}
+int exa_brick_model_load(const char *bricksPath, const char *scalarsPath, exa_brick_model **out) {
+  return guarded([&] {
+    require(bricksPath && scalarsPath && out, "invalid arguments");
+    auto m = std::make_unique<exa_brick_model>();
+    loadBrickModel(bricksPath, scalarsPath, m->m);
+    *out = m.release();
+  });
+}
+int exa_brick_model_from_amr(const exa_amr *a, int blockCells, exa_brick_model **out) {
+  return guarded([&] {
+    require(a && out, "invalid arguments");
+    std::vector<Brick> bricks;
+    std::vector<int32_t> ids;
+    bricksFromCells(a->scene.cells.data(), (int64_t)a->scene.cells.size(), blockCells, bricks, ids);
+    auto m = std::make_unique<exa_brick_model>();
+    brickModelFromCellIDs(bricks, ids, a->scene.scalars.data(), (int64_t)a->scene.scalars.size(), m->m);
+    *out = m.release();
+  });
+}
+int exa_amr_write_bricks(const exa_amr *a, int blockCells, const char *bricksPath, const char *scalarsPath) {
+  return guarded([&] {
+    require(a && bricksPath, "invalid arguments");
+    std::vector<Brick> bricks;
+    std::vector<int32_t> ids;
+    bricksFromCells(a->scene.cells.data(), (int64_t)a->scene.cells.size(), blockCells, bricks, ids);
+    writeBrickFile(bricksPath, bricks, ids);
+    if (scalarsPath) writeScalars(scalarsPath, a->scene.scalars);
+  });
+}
+int exa_brick_model_get_counts(const exa_brick_model *m, int64_t c[6]) {
+  return guarded([&] {
+    require(m && c, "invalid arguments");
+    c[0] = (int64_t)m->m.bricks.size();
+    c[1] = (int64_t)m->m.scalars.size();
+    c[2] = (int64_t)m->m.abrs.size();
+    c[3] = (int64_t)m->m.leafList.size();
+    c[4] = (int64_t)m->m.nodes.size();
+    c[5] = m->m.kdDepth;
+  });
+}
+int exa_brick_model_get(const exa_brick_model *m, exa_brick *bricks, float *scalars, exa_abr *abrs,
+                        int32_t *leafList) {
+  return guarded([&] {
+    require(m, "model is NULL");
+    const BrickModel &M = m->m;
+    if (bricks) memcpy(bricks, M.bricks.data(), M.bricks.size() * sizeof(Brick));
+    if (scalars) memcpy(scalars, M.scalars.data(), M.scalars.size() * sizeof(float));
+    if (abrs) memcpy(abrs, M.abrs.data(), M.abrs.size() * sizeof(Abr));
+    if (leafList) memcpy(leafList, M.leafList.data(), M.leafList.size() * sizeof(int32_t));
+  });
+}
+int exa_brick_model_get_kd_nodes(const exa_brick_model *m, int32_t *nodes) {
+  return guarded([&] {
+    require(m && nodes, "invalid arguments");
+    memcpy(nodes, m->m.nodes.data(), m->m.nodes.size() * sizeof(KdNode));
+  });
+}
+int exa_brick_model_get_bounds(const exa_brick_model *m, float cb[6], float vr[2]) {
+  return guarded([&] {
+    require(m && cb && vr, "invalid arguments");
+    memcpy(cb, m->m.cellBounds, 24);
+    memcpy(vr, m->m.valueRange, 8);
+  });
+}
+void exa_brick_model_destroy(exa_brick_model *m) { delete m; }
+
+int exa_brick_sampler_create(const exa_brick_model *m, void *stream, exa_brick_sampler **out) {
+  return guarded([&] {
+    require(m && out, "invalid arguments");
+    auto s = std::make_unique<exa_brick_sampler>();
+    s->build(m->m, (hipStream_t)stream);
+    *out = s.release();
+  });
+}
+int exa_brick_sampler_compute_max_opacities(exa_brick_sampler *s, const float *rgba, int numColors, float lo,
+                                            float hi, void *stream) {
+  return guarded([&] {
+    require(s, "sampler is NULL");
+    s->computeMaxOpacities(rgba, numColors, lo, hi, (hipStream_t)stream);
+  });
+}
+int exa_brick_sampler_get_max_opacities(const exa_brick_sampler *s, float *out) {
+  return guarded([&] {
+    require(s && out, "invalid arguments");
+    require(s->haveMaxOp, "max opacities not computed");
+    download(s->maxOp, out);
+  });
+}
+int exa_brick_sampler_locate(const exa_brick_sampler *s, const float *d_pos, int64_t n, exa_sample *d_out,
+                             void *stream) {
+  return guarded([&] {
+    require(s && n >= 0 && (n == 0 || (d_pos && d_out)), "invalid arguments");
+    launchBrickLocate(s->view(), d_pos, n, (Sample *)d_out, (hipStream_t)stream);
+    HIP_CHECK(hipGetLastError());
+  });
+}
+void exa_brick_sampler_destroy(exa_brick_sampler *s) { delete s; }
+const exa_brick_sampler *exa_renderer_get_brick_sampler(const exa_renderer *r) {
+  return r && r->bricks ? r->bricks.get() : nullptr;
 }
 
 }  // extern "C"

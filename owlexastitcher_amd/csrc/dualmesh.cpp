@@ -45,7 +45,7 @@ static void parallelChunks(int64_t numChunks, F &&f) {
 }
 
 // LSD radix sort of (key, payload) pairs, 16-bit digits; stable.
-static void radixSort(std::vector<uint64_t> &keys, std::vector<int64_t> &vals) {
+void radixSort(std::vector<uint64_t> &keys, std::vector<int64_t> &vals) {
   const size_t n = keys.size();
   std::vector<uint64_t> k2(n);
   std::vector<int64_t> v2(n);

@@ -99,6 +99,41 @@ EXA_HD void exa_sincosf(float x, float &sn, float &cs) {
   else { sn = -cr; cs = sr; }
 }
 
+// Own single-precision exp / pow (identical algorithm in the CPU oracle).
+//  expf: Cody & Waite reduction x = n ln2 + r (ln2 split as in exa_logf),
+//        Taylor polynomial of e^r to r^7 on |r| <= ln2/2, scaled by 2^n;
+//        results below the smallest normal float return 0.
+//  powf: x^y = exp(y log x) for x > 0, with pow(x, 0) = 1, pow(1, y) = 1,
+//        pow(0, y > 0) = 0.
+EXA_HD float exa_expf(float x) {
+  if (!(x == x)) return x;
+  if (x < -87.3f) return 0.f;
+  if (x > 88.72f) return __builtin_inff();
+  const float n = rintf(x * 1.44269502162933349609375f);
+  const float r = (x - n * 0.693145751953125f) - n * 1.428606765330187045e-06f;
+  const float p =
+      1.0f + r * (1.0f + r * (0.5f + r * (0.16666667163372039794921875f +
+                                          r * (0.0416666679084300994873046875f +
+                                               r * (0.008333333767950534820556640625f +
+                                                    r * (0.001388888922519981861114501953125f +
+                                                         r * 0.0001984127011382952332496643066406f))))));
+  int ni = (int)n;
+  float q = p;
+  if (ni > 127) {
+    q = q * 2.f;
+    ni -= 1;
+  }
+  return q * __builtin_bit_cast(float, (uint32_t)(ni + 127) << 23);
+}
+
+EXA_HD float exa_powf(float x, float y) {
+  if (y == 0.f) return 1.f;
+  if (x == 1.f) return 1.f;
+  if (x == 0.f) return y > 0.f ? 0.f : __builtin_inff();
+  if (!(x > 0.f)) return __builtin_nanf("");
+  return exa_expf(y * exa_logf(x));
+}
+
 // owl::common::LCG<4> (deviceCode.cu:41,1278): TEA seeding with 4 rounds
 // (Zafar, Olano, Curtis, HPG 2010), then the Numerical Recipes LCG; a sample
 // is the low 24 bits of the state scaled by 2^-24.

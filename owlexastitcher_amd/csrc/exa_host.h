@@ -12,6 +12,8 @@
 namespace exa {
 
 int hostThreads();
+// LSD radix sort of (key, payload) pairs by key, stable (dualmesh.cpp)
+void radixSort(std::vector<uint64_t> &keys, std::vector<int64_t> &vals);
 
 // ---------------------------------------------------------------------------
 // synthetic AMR scenes

@@ -7,7 +7,7 @@
 // (856-926), henyeyGreensteinSample (965-985), lookupTransferFunction
 // (320-341), and dda3 (DDA.h:26-113).  The persistent renderer
 // (render_persistent.hip) runs the same helpers as a resumable state machine.
-#include "exa_render_common.h"
+#include "exa_integrate.h"
 
 namespace exa {
 
@@ -20,13 +20,6 @@ struct Ctx {
   int32_t *stack;
   LocalCounters &cnt;
   int px, py;
-};
-
-struct Interaction {
-  int type;
-  vec3f pos;
-  float Tr;
-  float4 xf;
 };
 
 // deviceCode.cu:699-854: DDA over the majorant grid, Woodcock (shadow=false)
@@ -75,60 +68,6 @@ EXA_DEV Interaction sampleInteraction(Ctx<kCount, kAccel> &C, const Ray &ray, bo
   }
   I.pos = xfmPoint(P.voxelXfmInv, I.pos);
   return I;
-}
-
-template <bool kCount, int kAccel>
-EXA_DEV vec3f sampleLight(Ctx<kCount, kAccel> &C, vec3f pos, int numLights) {
-  int which;
-  float ld;
-  const Ray ray = lightRay(C.P, pos, numLights, C.rng, which, ld);
-  const Interaction I = sampleInteraction(C, ray, true);
-  return lightResult(C.P, which, I.Tr, ld);
-}
-
-template <bool kCount, int kAccel>
-EXA_DEV float4 pathTrace(Ctx<kCount, kAccel> &C, Ray ray, float4 bg, int numLights) {
-  const RenderParams &P = C.P;
-  vec3f throughput = mk3s(1.f);
-  vec3f Ld = mk3s(0.f);
-  unsigned bounce = 0;
-  float t0 = 1e30f, t1 = -1e30f;
-  if (boxTest(ray, P.worldBounds, t0, t1)) {
-    clipInterval(P, ray, t0, t1);
-    ray.org = ray.org + ray.dir * t0;
-    t1 -= t0;
-    for (;;) {
-      ray.tmax = t1;
-      const Interaction I = sampleInteraction(C, ray, false);
-      if (I.type == kBoundary) break;
-      if (!ptInteraction(P, I.xf, throughput, bounce, C.rng)) break;
-      if (numLights > 0) Ld = Ld + throughput * sampleLight(C, I.pos, numLights);
-      const vec3f scatterDir = hgSample(-ray.dir, C.rng);
-      ray.org = I.pos;
-      ray.dir = scatterDir;
-      boxTest(ray, P.worldBounds, t0, t1);
-      clipInterval(P, ray, t0, t1);
-    }
-  }
-  return ptResult(Ld, throughput, bounce, numLights, bg);
-}
-
-template <bool kCount, int kAccel>
-EXA_DEV float4 directLight(Ctx<kCount, kAccel> &C, Ray ray, float4 bg, int numLights) {
-  const RenderParams &P = C.P;
-  vec3f throughput = mk3s(1.f);
-  float t0 = 1e30f, t1 = -1e30f;
-  if (!boxTest(ray, P.worldBounds, t0, t1)) return bg;
-  clipInterval(P, ray, t0, t1);
-  ray.org = ray.org + ray.dir * t0;
-  t1 -= t0;
-  ray.tmax = t1;
-  const Interaction I = sampleInteraction(C, ray, false);
-  if (I.type == kBoundary) return bg;
-  const vec3f albedo = mk3(I.xf.x, I.xf.y, I.xf.z);
-  if (numLights > 0) throughput = throughput * (albedo * sampleLight(C, I.pos, numLights));
-  else throughput = throughput * albedo;
-  return make_float4(throughput.x, throughput.y, throughput.z, 1.f);
 }
 
 // Ray marcher (deviceCode.cu:1213-1260 rayMarchingIntegrate, 348-398

@@ -8,14 +8,16 @@ Follows viewer.cpp:119-158 (defaults), 755-885 (flags), 919-975 (renderer
 set-up), 1476-1513 (headless set-up) and headless.cpp:97-186 (frame loop, log
 and fps files), 250-298 (.xf file, xfRange).  The element mesh is this
 library's EXAELEM1 file (the reference reads a umesh file); ``-cells`` with
-``-scalars`` builds the dual mesh in memory instead of reading mesh files.
+``-scalars`` builds the dual mesh in memory instead of reading mesh files;
+``-bricks`` with ``-scalars`` renders with the ExaBricks sampler
+(OWLRenderer.cpp:145-148) when no mesh or grids file is given.
 
 Where the reference's headless binary parses a flag without applying it
 (-spp, --heat-map, -dt, --range), this harness applies it, as the reference's
 interactive viewer does through its widgets.  Two flags are additions:
 ``--frames N`` (frames rendered, default 510) and ``--screenshot-frame N``
-(default 10).  Flags that select samplers outside the ExaStitch path (-bricks,
--bm, -qc, -kdtree, -majorants, -mesh) are rejected.
+(default 10).  Flags that select samplers or modes that are not built (-bm,
+-qc, -kdtree, -majorants, -mesh) are rejected.
 """
 import datetime
 import shutil
@@ -27,10 +29,10 @@ import zlib
 
 import numpy as np
 
-from .host import AmrScene, DualMesh, ExaStitchModel, OWLRenderer, camera_look_at, xf_load
+from .host import AmrScene, DualMesh, ExaBrickModel, ExaStitchModel, OWLRenderer, camera_look_at, xf_load
 
 CLIP_PLANES_MAX = 1  # common.h:34
-UNSUPPORTED = {"-bricks": "ExaBricks", "-bm": "BigMesh", "-qc": "QuickClusters", "-kdtree": "k-d tree majorants",
+UNSUPPORTED = {"-bm": "BigMesh", "-qc": "QuickClusters", "-kdtree": "k-d tree majorants",
                "-majorants": "own majorants", "-mesh": "triangle meshes"}
 
 
@@ -42,6 +44,7 @@ class Cmdline:
         self.scalarFileName = ""
         self.gridsFileName = ""
         self.amrCellFileName = ""
+        self.exaBrickFileName = ""
         self.xfFileName = ""
         self.outFileName = "Witcher3.png"
         self.fpsFileName = "fps.log"
@@ -92,8 +95,10 @@ def parse(argv):
             c.gridsFileName = take(1, str)[0]
         elif a == "-cells":
             c.amrCellFileName = take(1, str)[0]
+        elif a == "-bricks":
+            c.exaBrickFileName = take(1, str)[0]
         elif a in UNSUPPORTED:
-            raise UsageError(f"'{a}' selects the {UNSUPPORTED[a]} sampler, which is not part of the ExaStitch path")
+            raise UsageError(f"'{a}' selects the {UNSUPPORTED[a]} mode, which is not built")
         elif a == "-xf":
             c.xfFileName = take(1, str)[0]
         elif a == "--range":
@@ -144,8 +149,8 @@ def parse(argv):
         else:
             raise UsageError(f"unknown cmdline arg '{a}'")
         i += 1
-    if not c.amrCellFileName and not c.inFileName and not c.gridsFileName:
-        raise UsageError("no model: give an element-mesh file and/or -grids, or -cells")
+    if not c.amrCellFileName and not c.inFileName and not c.gridsFileName and not c.exaBrickFileName:
+        raise UsageError("no model: give an element-mesh file and/or -grids, -bricks, or -cells")
     if c.frames < 1:
         raise UsageError("--frames must be at least 1")
     return c
@@ -294,7 +299,12 @@ class Headless:
 def build_renderer(c):
     """viewer.cpp:919-975 without the GUI: model, renderer, colour map, clip
     planes, integrator, shade mode, light."""
-    if c.amrCellFileName:
+    if c.exaBrickFileName and not (c.inFileName or c.gridsFileName):
+        # OWLRenderer.cpp:141-148: mesh or grids first, then bricks + scalars
+        if not c.scalarFileName:
+            raise UsageError("-bricks needs -scalars")
+        model = ExaBrickModel.load(c.exaBrickFileName, c.scalarFileName)
+    elif c.amrCellFileName:
         if not c.scalarFileName:
             raise UsageError("-cells needs -scalars")
         model = ExaStitchModel.from_dual_mesh(DualMesh(AmrScene.load(c.amrCellFileName, c.scalarFileName)))

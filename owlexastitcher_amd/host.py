@@ -4,6 +4,8 @@ Class and method names follow the reference so scripts read like its own
 host code:
   ExaStitchModel   model/ExaStitchModel.h:24-57 + model/Model.h:29-65
   ExaStitchSampler sampler/Sampler.h:28-69, sampler/ExaStitchSampler.h:37-106
+  ExaBrickModel    model/ExaBrickModel.h:27-56 + model/ABRs.h:28-100
+  ExaBrickSampler  sampler/ExaBrickSampler.h:30-241 (ABR mode)
   OWLRenderer      OWLRenderer.h:29-175
 Device buffers are torch tensors on the current HIP device (plumbing only);
 every computation runs in libexastitch.so.
@@ -99,6 +101,10 @@ class AmrScene:
 
     def save(self, cells_path=None, scalars_path=None):
         check(lib().exa_amr_save(self.h, _path(cells_path), _path(scalars_path)))
+
+    def write_bricks(self, bricks_path, scalars_path=None, block_cells=16):
+        """.bricks file of the brick maker's bricks (+ the .scalars file)"""
+        check(lib().exa_amr_write_bricks(self.h, int(block_cells), _path(bricks_path), _path(scalars_path)))
 
     def write_dual_mesh(self, element_mesh_path, cubes_prefix=None, grids_path=None):
         """amrMakeDualMesh + amrMakeGrids to files: the element mesh, the
@@ -324,16 +330,125 @@ class ExaStitchSampler:
             self.h = None
 
 
+BRICK_DTYPE = np.dtype([("lower", "<i4", 3), ("size", "<i4", 3), ("level", "<i4"), ("begin", "<u4")])
+ABR_DTYPE = np.dtype([("domain", "<f4", 6), ("valueRange", "<f4", 2), ("leafListBegin", "<i4"),
+                      ("leafListSize", "<i4"), ("finestLevelCellWidth", "<f4")])
+
+
+class ExaBrickModel:
+    """ExaBrickModel: bricks of AMR cells + their active brick regions."""
+
+    def __init__(self, handle):
+        self.h = handle
+
+    @classmethod
+    def load(cls, brick_file_name, scalar_file_name):
+        """ExaBrickModel::load(brickFile, scalarFile, "") (ExaBrickModel.cpp:27-117)"""
+        h = C.c_void_p()
+        check(lib().exa_brick_model_load(_path(brick_file_name), _path(scalar_file_name), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def from_arrays(cls, bricks, scalars):
+        """ExaBrickModel::load(bricks, scalars, numBricks) (ExaBrickModel.cpp:119-145)"""
+        b = np.ascontiguousarray(bricks, BRICK_DTYPE)
+        s = _f32(scalars).reshape(-1)
+        h = C.c_void_p()
+        check(lib().exa_brick_model_create(_ptr(b), _ptr(s), len(b), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def from_amr(cls, amr, block_cells=16):
+        """bricks of an AMR scene's leaf cells (the brick maker)"""
+        h = C.c_void_p()
+        check(lib().exa_brick_model_from_amr(amr.h, int(block_cells), C.byref(h)))
+        return cls(h)
+
+    def counts(self):
+        c = np.zeros(6, np.int64)
+        check(lib().exa_brick_model_get_counts(self.h, _ptr(c)))
+        return dict(zip(("bricks", "cells", "abrs", "leaf_list", "kd_nodes", "kd_depth"), (int(x) for x in c)))
+
+    def arrays(self):
+        """(bricks [BRICK_DTYPE], scalars, abrs [ABR_DTYPE], leafList)"""
+        c = self.counts()
+        b = np.empty(c["bricks"], BRICK_DTYPE)
+        s = np.empty(c["cells"], np.float32)
+        a = np.empty(c["abrs"], ABR_DTYPE)
+        ll = np.empty(c["leaf_list"], np.int32)
+        check(lib().exa_brick_model_get(self.h, _ptr(b), _ptr(s), _ptr(a), _ptr(ll)))
+        return b, s, a, ll
+
+    def kd_nodes(self):
+        """(split float32, info int32) per kd node"""
+        n = np.empty((self.counts()["kd_nodes"], 2), np.int32)
+        check(lib().exa_brick_model_get_kd_nodes(self.h, _ptr(n)))
+        return n[:, 0].view(np.float32).copy(), n[:, 1].copy()
+
+    def bounds(self):
+        cb = np.zeros(6, np.float32)
+        vr = np.zeros(2, np.float32)
+        check(lib().exa_brick_model_get_bounds(self.h, _ptr(cb), _ptr(vr)))
+        return cb, vr
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().exa_brick_model_destroy(self.h)
+            self.h = None
+
+
+class ExaBrickSampler:
+    """ExaBrickSampler in ABR mode: computeMaxOpacities / device sample()."""
+
+    def __init__(self, model=None, stream=None, handle=None, owner=None):
+        self._owned = handle is None
+        if handle is None:
+            handle = C.c_void_p()
+            check(lib().exa_brick_sampler_create(model.h, _stream(stream), C.byref(handle)))
+        self.h = handle
+        self._owner = owner
+        self._n = None if model is None else model.counts()["abrs"]
+
+    def computeMaxOpacities(self, rgba, xf_range, stream=None):
+        cm = _f32(rgba).reshape(-1, 4)
+        check(lib().exa_brick_sampler_compute_max_opacities(self.h, _ptr(cm), len(cm), float(xf_range[0]),
+                                                            float(xf_range[1]), _stream(stream)))
+
+    def max_opacities(self, num_abrs=None):
+        n = num_abrs if num_abrs is not None else self._n
+        out = np.empty(n, np.float32)
+        check(lib().exa_brick_sampler_get_max_opacities(self.h, _ptr(out)))
+        return out
+
+    def locate(self, d_pos, d_out, stream=None):
+        """d_pos: float32 tensor (n,3) on the GPU; d_out: int32 tensor (n,3)
+        viewed as Sample records: {0, -1, value} or {0, 0, 0}."""
+        check(lib().exa_brick_sampler_locate(self.h, C.c_void_p(d_pos.data_ptr()), d_pos.shape[0],
+                                             C.c_void_p(d_out.data_ptr()), _stream(stream)))
+
+    def __del__(self):
+        if self._owned and getattr(self, "h", None):
+            lib().exa_brick_sampler_destroy(self.h)
+            self.h = None
+
+
 class OWLRenderer:
-    """OWLRenderer: model -> sampler -> Woodcock path-traced frames."""
+    """OWLRenderer: model -> sampler -> Woodcock path-traced frames.  An
+    ExaBrickModel gives the ExaBricks renderer (OWLRenderer.cpp:269-291)."""
 
     def __init__(self, model, num_mcs=(128, 128, 128), remap_from=None, remap_to=None):
         self.h = C.c_void_p()
         mcs = np.ascontiguousarray(num_mcs, dtype=np.int32)
         rf = _f32(remap_from, 6) if remap_from is not None else None
         rt = _f32(remap_to, 6) if remap_to is not None else None
-        check(lib().exa_renderer_create(model.h, _ptr(mcs), _ptr(rf), _ptr(rt), C.byref(self.h)))
-        self.sampler = ExaStitchSampler(lib().exa_renderer_get_sampler(self.h), owner=self)
+        self.bricks = isinstance(model, ExaBrickModel)
+        if self.bricks:
+            check(lib().exa_renderer_create_bricks(model.h, _ptr(mcs), _ptr(rf), _ptr(rt), C.byref(self.h)))
+            self.sampler = ExaBrickSampler(handle=lib().exa_renderer_get_brick_sampler(self.h), owner=self)
+            self.sampler._n = model.counts()["abrs"]
+        else:
+            check(lib().exa_renderer_create(model.h, _ptr(mcs), _ptr(rf), _ptr(rt), C.byref(self.h)))
+            self.sampler = ExaStitchSampler(lib().exa_renderer_get_sampler(self.h), owner=self)
 
     # --- OWLRenderer.h:88-149 ---
     def setCamera(self, org, dir_00, dir_du, dir_dv):
@@ -352,6 +467,9 @@ class OWLRenderer:
         ptr = C.c_void_p(d_fb.data_ptr()) if d_fb is not None else None
         c = np.zeros(11, np.uint64)
         check(lib().exa_renderer_render_counted(self.h, ptr, _ptr(c), _stream(stream)))
+        if self.bricks:
+            return {"samples": int(c[0]), "bricks": int(c[1]), "kd_leaves": int(c[2]), "regions": int(c[7]),
+                    "pixels": int(c[8]), "tf": int(c[9])}
         names = ("gridlet", "tet", "pyr", "wedge", "hex", "miss", "cached", "dda_cells", "pixels", "tf", "neural")
         return dict(zip(names, (int(x) for x in c)))
 

@@ -17,9 +17,10 @@ GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 def files(tmp_path_factory):
     d = tmp_path_factory.mktemp("headless")
     a = AmrScene.noise(16, 2, 0.55, 0.7, 6, 2.2, 5)
-    paths = {k: str(d / f"m.{k}") for k in ("exum", "grids", "scalars", "cells")}
+    paths = {k: str(d / f"m.{k}") for k in ("exum", "grids", "scalars", "cells", "bricks")}
     a.write_dual_mesh(paths["exum"], None, paths["grids"])
     a.save(paths["cells"], paths["scalars"])
+    a.write_bricks(paths["bricks"], None, 8)
     return d, paths
 
 
@@ -64,3 +65,17 @@ def test_headless_run_from_cells(files, device):
     st = s["renderer"].state()
     assert st.integrator == 2 and st.shadeMode == 1 and st.dt == 0.5 and st.clipEnabled == 1
     assert st.lightOn[0] == 1 and os.path.exists(out + ".png") and s["timed_frames"] == 2
+
+
+def test_headless_run_with_bricks(files, device):
+    d, p = files
+    out = str(d / "run3")
+    print("case: headless -bricks, direct lighting, 32x32, 3 frames", flush=True)
+    s = hl.run(["-bricks", p["bricks"], "-scalars", p["scalars"], "--size", "32", "32", "-o", out, "-fps",
+                str(d / "fps3.log"), "--frames", "3", "--screenshot-frame", "1", "-rt", "1",
+                "-xf", os.path.join(GOLDEN, "cloud.xf")])
+    r = s["renderer"]
+    assert r.bricks and r.state().integrator == 1
+    assert os.path.exists(out + ".png") and s["timed_frames"] == 2
+    fb = s["viewer"].fb.cpu().numpy().view(np.uint32)
+    assert len(np.unique(fb)) > 10

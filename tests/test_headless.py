@@ -40,11 +40,13 @@ def test_flags():
     assert c.valueRange == (0.1, 0.9) and c.subImage == (0, 0, 0.5, 0.5)
     assert (c.frames, c.screenshotFrame) == (20, 3)
     assert hl.parse(["-cells", "a.cells", "-scalar", "a.scalars"]).amrCellFileName == "a.cells"
+    assert hl.parse(["-bricks", "a.bricks", "-scalars", "a.scalars"]).exaBrickFileName == "a.bricks"
 
 
 @pytest.mark.parametrize("argv,msg", [
     (["m.exum", "--bogus"], "unknown cmdline arg"),
-    (["m.exum", "-bricks", "b"], "ExaBricks"),
+    (["m.exum", "-bm", "b"], "BigMesh"),
+    (["m.exum", "-kdtree", "k"], "k-d tree"),
     (["m.exum", "--clip-plane", "0", "0", "1", "1", "--clip-plane", "0", "0", "1", "2"], "at most 1"),
     (["m.exum", "--camera", "1", "2"], "missing value"),
     ([], "no model"),
