@@ -85,40 +85,103 @@ EXA_DEV bool abrBoxTest(const BrickView &v, int32_t abr, vec3f org, vec3f dir, f
   return t0 < t1;
 }
 
-// The closest hit of a ray from tmin among the regions not culled
+// The closest hit of a ray from tq among the regions not culled
 // (ExaBrickABRGeomIsect, sampler/programs/ExaBrickSampler.cu:45-67, under
-// deviceCode.cu:527-547 traverse): the first region, in front-to-back kd
-// order, whose box test passes.  Each step descends from the root to the
-// leaf holding the walk parameter ta (toward the lower side where ta is
-// strictly before the plane, else the upper side), whose kd interval ends at
-// tb > ta; the next step starts at tb.  A region whose box test passes lies
-// on the near side of every plane at which its entry is later, so the walk
-// reaches it before any region entered later.  Returns the region, or -1.
-EXA_DEV int32_t nextAbr(const BrickView &v, vec3f org, vec3f dir, float tmin, float tmax, float &t0, float &t1,
-                        uint32_t &leaves) {
-  float ta = tmin;
-  for (int guard = 0; guard < kMaxAbrVisits && ta < tmax; ++guard) {
-    int32_t node = 0;
-    float tb = tmax;
-    int2 n = v.nodes[0];
+// deviceCode.cu:527-547 traverse), answered by walking the kd-tree front to
+// back: the first region whose box test from tq passes.
+//
+// Restart form: each step descends from the root to the leaf holding the
+// walk parameter ta (toward the lower side where ta is strictly before the
+// plane, else the upper side, by the direction's sign), whose kd interval
+// ends at tb > ta; the next step starts at tb.  A region whose box test
+// passes lies on the near side of every plane at which its entry is later,
+// so the walk reaches it before any region entered later.
+//
+// The walk below gives the same leaves without the restarts and keeps its
+// place between queries.  A descent pushes the far child of every plane it
+// passes on the near side while the far side is still inside the interval
+// (the entry keeps the interval's end); the planes of the entries strictly
+// decrease toward the top, and the top's plane is the exit of the leaf
+// reached, so popping it continues exactly where a restart from the root at
+// that exit would go.  The stack holds kWalkStack entries; when it runs empty
+// the walk restarts at the root (needed only after an entry had to be
+// dropped).  A query from tq after a region [t0, t1] returned at tq >= t1
+// continues at that region's exit: the leaves in between are that region
+// (whose box test fails from tq) or lie before tq.
+constexpr int kWalkStack = 6;
+struct AbrWalk {
+  float ta, tb, tmax;
+  int32_t node;
+  int32_t sp;
+  bool dropped;
+  int32_t sn[kWalkStack];
+  float st[kWalkStack];
+};
+
+EXA_DEV void walkBegin(AbrWalk &w, float tmin, float tmax) {
+  w.ta = tmin;
+  w.tb = tmax;
+  w.tmax = tmax;
+  w.node = 0;
+  w.sp = 0;
+  w.dropped = false;
+}
+
+// The first region not culled whose box test from tq passes, or -1.
+EXA_DEV int32_t walkNext(const BrickView &v, AbrWalk &w, vec3f org, vec3f dir, float tq, float &t0, float &t1,
+                         uint32_t &leaves) {
+  for (int guard = 0; guard < kMaxAbrVisits && w.ta < w.tmax; ++guard) {
+    int32_t node = w.node;
+    float tb = w.tb;
+    const float ta = w.ta;
+    int2 n = v.nodes[node];
     while ((n.y & 3) != 3) {
       const int dim = n.y & 3;
       const float d = comp(dir, dim);
       const float ts = (__int_as_float(n.x) - comp(org, dim)) / d;
       const int32_t lowerChild = n.y >> 2, upperChild = node + 1;
       const bool neg = signbit(d);
+      const int32_t nearC = neg ? upperChild : lowerChild, farC = neg ? lowerChild : upperChild;
       if (ta < ts) {
-        node = neg ? upperChild : lowerChild;
-        tb = fminf(tb, ts);
+        if (ts < tb) {
+          // push (farC, tb); the stack's top is entry 0 (static register moves)
+          if (w.sp == kWalkStack) w.dropped = true;
+          else ++w.sp;
+#pragma unroll
+          for (int k = kWalkStack - 1; k > 0; --k) {
+            w.sn[k] = w.sn[k - 1];
+            w.st[k] = w.st[k - 1];
+          }
+          w.sn[0] = farC;
+          w.st[0] = tb;
+          tb = ts;
+        }
+        node = nearC;
       } else {
-        node = neg ? lowerChild : upperChild;
+        node = farC;
       }
       n = v.nodes[node];
     }
     ++leaves;
+    // the continuation after this leaf, whose kd interval ends at tb
+    w.ta = tb;
+    if (w.sp > 0) {
+      w.node = w.sn[0];
+      w.tb = w.st[0];
+#pragma unroll
+      for (int k = 0; k < kWalkStack - 1; ++k) {
+        w.sn[k] = w.sn[k + 1];
+        w.st[k] = w.st[k + 1];
+      }
+      --w.sp;
+    } else {
+      // an empty stack with nothing dropped leaves tb == tmax: the loop ends
+      w.node = 0;
+      w.tb = w.tmax;
+      w.dropped = false;
+    }
     const int32_t abr = (n.y >> 2) - 1;
-    if (abr >= 0 && !abrCulled(v, abr) && abrBoxTest(v, abr, org, dir, tmin, tmax, t0, t1)) return abr;
-    ta = tb;
+    if (abr >= 0 && !abrCulled(v, abr) && abrBoxTest(v, abr, org, dir, tq, w.tmax, t0, t1)) return abr;
   }
   return -1;
 }

@@ -57,9 +57,11 @@ EXA_DEV Interaction sampleInteraction(BrickCtx<kCount> &C, const Ray &ray, bool 
   const float tmax = ray.tmax * dt_scale;
   float already = ray.tmin * dt_scale;
   uint32_t leaves = 0, bricks = 0;
+  AbrWalk walk;
+  walkBegin(walk, already, tmax);
   for (int visit = 0; visit < kMaxAbrVisits; ++visit) {
     float t0, t1;
-    const int32_t abr = nextAbr(C.bv, org, dir, already, tmax, t0, t1, leaves);
+    const int32_t abr = walkNext(C.bv, walk, org, dir, already, t0, t1, leaves);
     if (abr < 0) break;
     if (kCount) C.cnt.c[kCntDdaCells]++;
     const float majorant = C.bv.abrMaxOp[abr];
@@ -120,9 +122,11 @@ EXA_DEV float4 rayMarchBricks(BrickCtx<kCount> &C, Ray ray, float4 bg) {
   float4 pc = make_float4(0.f, 0.f, 0.f, 0.f);
   uint32_t leaves = 0, bricks = 0;
   int steps = 0;
+  AbrWalk walk;
+  walkBegin(walk, already, tmax);
   for (int visit = 0; visit < kMaxAbrVisits && steps < kMaxBrickMarchSteps; ++visit) {
     float t0, t1;
-    const int32_t abr = nextAbr(C.bv, org, dir, already, tmax, t0, t1, leaves);
+    const int32_t abr = walkNext(C.bv, walk, org, dir, already, t0, t1, leaves);
     if (abr < 0) break;
     if (kCount) C.cnt.c[kCntDdaCells]++;
     const float dt = P.dt * C.bv.abrCellWidth[abr];
