@@ -163,3 +163,36 @@ def test_render_counted_and_state_errors(device):
     from owlexastitcher_amd import ExaError
     with pytest.raises(ExaError):
         r.setAccel(1)
+
+
+@pytest.mark.parametrize("integrator", [0, 2])
+def test_render_clip_remap_spp(device, integrator):
+    """The brick renderer with a voxel-space remap (Model::setVoxelSpaceTransform),
+    a clip plane and 2 spp per frame, against the oracle."""
+    print(f"case: brick render integrator={integrator} remap+clip+2spp 40x32", flush=True)
+    model = ExaBrickModel.from_amr(_scene("noise"), 8)
+    cb, _ = model.bounds()
+    remap_from = cb
+    remap_to = np.concatenate([cb[:3] * 0.5 + 3.0, cb[3:] * 0.5 + 3.0]).astype(np.float32)
+    r = OWLRenderer(model, (16, 16, 16), remap_from, remap_to)
+    W, H = 40, 32
+    b = r.modelBounds
+    center = 0.5 * (b[:3] + b[3:])
+    span = b[3:] - b[:3]
+    org, d00, du, dv = camera_look_at(center + np.array([0.6, 0.5, -1.1]) * span, center, (0, 1, 0), 55.0, W, H)
+    r.resize(W, H)
+    r.setCamera(org, d00, du, dv)
+    r.setColorMap(scenes.tf128())
+    vr = r.valueRange
+    r.setRange(vr[0], vr[1])
+    r.setRelDomain(0.0, 100.0)
+    r.setOpacityScale(118.0)
+    r.setType(integrator)
+    r.setMaxBounces(3)
+    r.setSpp(2)
+    r.setClipPlane(0, True, (0.3, 0.0, 1.0), float(0.3 * center[0] + center[2]))
+    r.setDirectionalLight(0, (0.2, 1.0, 0.4), 1.5, True)
+    acc, acc_o, fb, fb_o = render_both(model, r, 2, device)
+    assert np.array_equal(acc, acc_o), _report(acc, acc_o)
+    assert np.array_equal(fb, fb_o)
+    assert not np.all(acc == acc[:, :1])
