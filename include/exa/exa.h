@@ -293,6 +293,10 @@ typedef struct {
   float dt; /* ray-marcher step */
   int32_t heatMapEnabled;
   float heatMapScale;
+  int32_t subImageActive; /* setSubImage: lower.xy, upper.xy in [0,1] */
+  float subImage[4];
+  int32_t subImageSelecting; /* setSubImageSelection */
+  float subImageSel[4];
 } exa_render_state;
 int exa_renderer_get_state(const exa_renderer *r, exa_render_state *out);
 void exa_renderer_destroy(exa_renderer *r);

@@ -177,6 +177,12 @@ typedef struct {
   /* ExaBricks scene (renderFrame_ExaBrickSampler, deviceCode.cu:1378-1388);
      NULL: the exaStitch model passed to the render call */
   const struct ora_brick_scene *bricks;
+  /* deviceCode.cu:1285-1296 sub-image (lower.xy, upper.xy in [0,1]) and
+     1337-1340 selection box */
+  int32_t subImageActive;
+  float   subImage[4];
+  int32_t subImageSelecting;
+  float   subImageSel[4];
 } ora_render_params;
 
 /* Render rows [y0,y1) of one frame.  accum is float4[W*H] (in/out),

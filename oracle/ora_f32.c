@@ -509,7 +509,18 @@ void ora_render_f32(const ora_render_params *p, const ora_model *m, const ora_bi
       a[0] = acc.x; a[1] = acc.y; a[2] = acc.z; a[3] = acc.w;
       const float sc = 1.f / (float)(p->accumID + 1);
       acc.x *= sc; acc.y *= sc; acc.z *= sc; acc.w *= sc;
-      if (p->debugCrosshair && (px == p->W / 2 || py == p->H / 2)) {
+      int invert = p->debugCrosshair && (px == p->W / 2 || py == p->H / 2);
+      if (p->subImageSelecting) {
+        /* deviceCode.cu:134-141 subImageSelectionWin, 1337-1340: the border
+           of the selection box */
+        const int lx = clampi(sat_i32(p->subImageSel[0] * (float)p->W), 0, p->W - 1);
+        const int ly = clampi(sat_i32(p->subImageSel[1] * (float)p->H), 0, p->H - 1);
+        const int hx = clampi(sat_i32(p->subImageSel[2] * (float)p->W), 0, p->W - 1);
+        const int hy = clampi(sat_i32(p->subImageSel[3] * (float)p->H), 0, p->H - 1);
+        invert = invert || (((px == lx || px == hx) && (py >= ly && py <= hy)) ||
+                            ((py == ly || py == hy) && (px >= lx && px <= hx)));
+      }
+      if (invert) {
         acc.x = 1.f - acc.x; acc.y = 1.f - acc.y; acc.z = 1.f - acc.z; acc.w = 1.f - acc.w;
       }
       if (fb) {

@@ -45,6 +45,8 @@ class OraRenderParams(C.Structure):
         ("lights", OraLight * 4),
         ("debugCrosshair", C.c_int32), ("shadeMode", C.c_int32), ("dt", C.c_float),
         ("bricks", C.c_void_p),
+        ("subImageActive", C.c_int32), ("subImage", C.c_float * 4),
+        ("subImageSelecting", C.c_int32), ("subImageSel", C.c_float * 4),
     ]
 
 
@@ -347,10 +349,10 @@ def render_params_from_state(st, majorants, tf_rgba):
     """Fill oracle render params from an exa_render_state snapshot."""
     p = OraRenderParams()
     for f in ("W", "H", "spp", "accumID", "integrator", "maxBounces", "opacityScale", "clipEnabled", "clipD",
-              "tfN", "debugCrosshair", "shadeMode", "dt"):
+              "tfN", "debugCrosshair", "shadeMode", "dt", "subImageActive", "subImageSelecting"):
         setattr(p, f, getattr(st, f))
     for f in ("camOrg", "camDir00", "camDu", "camDv", "worldBounds", "voxelXfm", "voxelXfmInv", "lightXfm",
-              "gridDims", "gridBounds", "tfDomain", "clipN"):
+              "gridDims", "gridBounds", "tfDomain", "clipN", "subImage", "subImageSel"):
         C.memmove(getattr(p, f), getattr(st, f), C.sizeof(getattr(p, f)))
     for i in range(4):
         for k in range(3):

@@ -70,6 +70,8 @@ class RenderState(C.Structure):
         ("lightPos", (C.c_float * 3) * 4), ("lightIntensity", C.c_float * 4),
         ("lightOn", C.c_int32 * 4), ("lightType", C.c_int32 * 4),
         ("dt", C.c_float), ("heatMapEnabled", C.c_int32), ("heatMapScale", C.c_float),
+        ("subImageActive", C.c_int32), ("subImage", C.c_float * 4),
+        ("subImageSelecting", C.c_int32), ("subImageSel", C.c_float * 4),
     ]
 
 

@@ -1962,6 +1962,10 @@ int exa_renderer_get_state(const exa_renderer *r, exa_render_state *o) {
     o->dt = P.dt;
     o->heatMapEnabled = P.heatMapEnabled;
     o->heatMapScale = P.heatMapScale;
+    o->subImageActive = P.subImageActive;
+    memcpy(o->subImage, P.subImage, 16);
+    o->subImageSelecting = P.subImageSelecting;
+    memcpy(o->subImageSel, P.subImageSel, 16);
   });
 }
 void exa_renderer_destroy(exa_renderer *r) { delete r; }

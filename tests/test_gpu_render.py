@@ -209,3 +209,20 @@ def test_persistent_matches_megakernel_noise(device):
     assert c0 == c1, (c0, c1)
     assert c0["hex"] > 0 and c0["wedge"] > 0 and c0["pyr"] > 0
     assert c0["pixels"] == 160 * 120
+
+
+@pytest.mark.parametrize("scheme", SCHEMES, ids=["persistent", "megakernel"])
+def test_sub_image_and_selection_match_oracle(cfg1, device, scheme):
+    """setSubImage / setSubImageSelection (deviceCode.cu:1285-1296, 1337-1340):
+    the frame of a sub-image window with the selection box drawn, bit for bit
+    against the oracle."""
+    print(f"case: sub-image + selection, scheme {scheme}, 64x48", flush=True)
+    r = make_renderer(cfg1, 64, 48)
+    r.setScheme(scheme)
+    r.setMaxBounces(4)
+    r.setDirectionalLight(0, (0.3, 1.0, 0.5), 2.0, True)
+    r.setSubImage((0.15, 0.2, 0.65, 0.8), True)
+    r.setSubImageSelection((0.1, 0.25, 0.6, 0.75), True)
+    acc, acc_o, fb, fb_o = render_both(cfg1, r, 2, device)
+    assert np.array_equal(acc, acc_o), _report(acc, acc_o)
+    assert np.array_equal(fb, fb_o)

@@ -209,3 +209,58 @@ def test_teaser_blends_default_and_gridlets():
     hi = np.maximum(img[0], img[1])[mid]
     assert np.all(img[2][mid] >= lo - 1e-6) and np.all(img[2][mid] <= hi + 1e-6)
     assert np.array_equal(img[2][..., 3], img[0][..., 3])
+
+
+def test_sub_image_window_and_selection_box():
+    """renderFrame_Impl's sub-image (deviceCode.cu:1285-1296): pixel (x, y)
+    with jitter (rx, ry) traces the ray through image position
+    ((1-u) lo + u hi) * W with u = (x + rx) / W (likewise y): with direct
+    lighting, no lights and a huge extinction a pixel shows the TF colour
+    where that ray crosses the box and the background where it misses.  The
+    selection box (1337-1340) inverts exactly the frame-buffer pixels on its
+    border."""
+    om = scene()
+    col = np.array([0.3, 0.6, 0.9], np.float32)
+    p = params(om, const_tf(col, 1.0), 1, os_=1e4)
+    p.subImageActive = 1
+    lo, hi = (-0.2, -0.1), (0.45, 0.6)  # a window over the box's upper-left silhouette
+    p.subImage[0], p.subImage[1], p.subImage[2], p.subImage[3] = lo[0], lo[1], hi[0], hi[1]
+    acc, fb = ora.render_f32(p, om)
+    cb, _ = om.bounds()
+    blo, bhi = cb[:3].astype(np.float64), cb[3:].astype(np.float64)
+    bg = background(H_IMG)
+    inside = miss = 0
+    for y in range(H_IMG):
+        for x in range(W_IMG):
+            rx, ry = ora.lcg_stream(x + W_IMG * y, 0, 2)
+            u = np.float32(np.float32(x) + rx) / np.float32(W_IMG)
+            v = np.float32(np.float32(y) + ry) / np.float32(H_IMG)
+            sx = ((np.float32(1) - u) * np.float32(lo[0]) + u * np.float32(hi[0])) * np.float32(W_IMG)
+            sy = ((np.float32(1) - v) * np.float32(lo[1]) + v * np.float32(hi[1])) * np.float32(H_IMG)
+            d = pixel_rays(p, np.array([[float(sx)]]), np.array([[float(sy)]]))
+            t0, t1 = (z[0, 0] for z in slab(EYE, d, blo, bhi))
+            if t1 > t0 + 0.05:
+                assert np.allclose(acc[y, x, :3], col, atol=1e-6), (y, x)
+                inside += 1
+            elif t1 < t0 - 0.05:
+                assert np.allclose(acc[y, x, :3], bg[y, x], atol=1e-6), (y, x)
+                miss += 1
+    assert inside > 100 and miss > 100, (inside, miss)
+    # the selection box inverts its border in the frame buffer only
+    p.subImageSelecting = 1
+    sel = (0.1, 0.25, 0.6, 0.75)
+    for k in range(4):
+        p.subImageSel[k] = sel[k]
+    p.debugCrosshair = 0
+    acc2, fb2 = ora.render_f32(p, om)
+    assert np.array_equal(acc2, acc)
+    lx, ly = int(np.float32(sel[0]) * np.float32(W_IMG)), int(np.float32(sel[1]) * np.float32(H_IMG))
+    hx, hy = int(np.float32(sel[2]) * np.float32(W_IMG)), int(np.float32(sel[3]) * np.float32(H_IMG))
+    ys, xs = np.mgrid[0:H_IMG, 0:W_IMG]
+    border = (((xs == lx) | (xs == hx)) & (ys >= ly) & (ys <= hy)) | (((ys == ly) | (ys == hy)) & (xs >= lx) & (xs <= hx))
+    val = acc.astype(np.float32)  # accumID 0, spp 1: the pixel values
+    val = np.where(border[..., None], np.float32(1) - val, val)
+    q = np.clip((val * np.float32(256)).astype(np.int64), 0, 255).astype(np.uint32)
+    want = q[..., 0] | (q[..., 1] << 8) | (q[..., 2] << 16) | (q[..., 3] << 24)
+    assert np.array_equal(fb2, want)
+    assert border.sum() > 20 and not np.array_equal(fb2[border], fb[border])
