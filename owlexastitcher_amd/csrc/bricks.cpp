@@ -44,6 +44,10 @@ int64_t brickCells(const Brick &b) { return (int64_t)b.size[0] * b.size[1] * b.s
 void checkBrick(const Brick &b) {
   if (b.size[0] <= 0 || b.size[1] <= 0 || b.size[2] <= 0) throw std::runtime_error("brick with an empty size");
   if (b.level < 0 || b.level > 30) throw std::runtime_error("brick level out of range");
+  for (int k = 0; k < 3; ++k) {
+    const int64_t hi = (int64_t)b.lower[k] + ((int64_t)b.size[k] << b.level);
+    if (hi > INT32_MAX) throw std::runtime_error("brick extends beyond the int32 coordinate range");
+  }
 }
 
 // A brick's domain clipped to a region (ABRs.cpp buildPrims)
@@ -91,6 +95,9 @@ struct AbrBuilder {
   // side's subtree, then the lower side's (the reference recurses into the
   // upper side first, which fixes the ABR IDs)
   void build(std::vector<Prim> &prims, const float dom[6], int32_t depth) {
+    // every split separates two distinct face planes of the bricks, so the
+    // depth stays far below this; the bound only keeps the host stack safe
+    if (depth > 4096) throw std::runtime_error("active brick regions nested too deeply");
     const int32_t self = (int32_t)m.nodes.size();
     m.nodes.push_back(KdNode{0.f, leafInfo(-1)});
     maxDepth = std::max(maxDepth, depth);

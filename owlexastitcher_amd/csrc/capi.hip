@@ -1131,6 +1131,7 @@ static void doRender(exa_renderer *r, uint32_t *fb, unsigned long long *counters
   P.tf = r->tf.ptr;
   P.tfN = (int32_t)r->tf.count;
   if (r->bricks) {
+    if (!r->bricks->haveMaxOp) throw Status(EXA_ERR_STATE, "region max opacities not computed");
     launchRenderBricks(P, r->bricks->view(), counters, s);
     HIP_CHECK(hipGetLastError());
     r->accumID++;

@@ -307,6 +307,22 @@ def test_brick_file_round_trip_and_reference_layout(tmp_path):
         ExaBrickModel.load(str(tmp_path / "exa-test-data.bricks"), str(tmp_path / "short.scalar"))
 
 
+def test_invalid_bricks_rejected():
+    """Bricks the host refuses before any region is built: empty sizes,
+    levels past 30, and extents beyond the int32 coordinate range (the
+    device computes lower + size << level in int32)."""
+    from owlexastitcher_amd import ExaError
+    s = np.zeros(8, np.float32)
+    for lower, size, level in [((0, 0, 0), (0, 2, 2), 0), ((0, 0, 0), (2, 2, 2), 31),
+                               ((2**31 - 8, 0, 0), (2, 2, 2), 2), ((0, 0, 0), (2, 2, 2), 30)]:
+        b = np.array([(lower, size, level, 0)], ora.BRICK_DTYPE)
+        with pytest.raises(ExaError):
+            ExaBrickModel.from_arrays(b, s)
+    # the largest extent that still fits: upper face at 2^31 - 4
+    b = np.array([((2**31 - 8, 0, 0), (2, 2, 2), 1, 0)], ora.BRICK_DTYPE)
+    assert ExaBrickModel.from_arrays(b, s).counts()["bricks"] == 1
+
+
 # ---- renders -----------------------------------------------------------------
 W_IMG, H_IMG = 40, 32
 
