@@ -6,7 +6,8 @@ gridlets), 64^3 majorant grid, path tracer capped at 4 bounces, one
 directional light, the 5-point TF at opacity scale 100.  The majorant grid and
 the per-primitive culling come from the oracle's own build and are checked
 against the library's; then two frames of each launch scheme are compared with
-the oracle's accumulation buffer and RGBA8 frame buffer, every pixel.
+the oracle's accumulation buffer and RGBA8 frame buffer, every pixel.  The
+locate and batched-requests workloads follow at their full sizes.
 """
 import numpy as np
 import pytest
@@ -80,3 +81,87 @@ def test_timed_frame_matches_oracle(workload, device, scheme):
     assert np.array_equal(fb.cpu().numpy().view(np.uint32), fb_o)
     # the view shows the volume, not only the background
     assert np.count_nonzero(np.any(acc != acc[:1, :1], axis=-1)) > W * H // 10
+
+
+def test_locate_workload_matches_oracle(workload, device):
+    """The sampler-alone workload: 2^24 points uniformly random over the model
+    box (the point set of the locate benchmark, seed 1), located and
+    interpolated under the TF's culling, against the oracle's binned search."""
+    from concurrent.futures import ThreadPoolExecutor
+    from tests.test_gpu_sampler import VALUE_RTOL, _compare, _locate_gpu
+    r, om, _, _ = workload
+    n = 1 << 24
+    print(f"case: locate workload, {n} uniform points, seed 1", flush=True)
+    b = r.modelBounds
+    pos = np.random.default_rng(1).uniform(b[:3], b[3:], (n, 3)).astype(np.float32)
+    ids, vals = _locate_gpu(r.sampler, pos, device)
+    chunk = 1 << 20
+    om.bins()  # built once, before the threads share it
+    with ThreadPoolExecutor(16) as ex:
+        ref = np.concatenate(list(ex.map(lambda i: om.locate_binned(pos[i:i + chunk]), range(0, n, chunk))))
+    mism, rel, ndiff = _compare(ids, vals, ref)
+    assert mism == 0 and rel <= VALUE_RTOL, (mism, rel, ndiff)
+    hit = ref[:, 0] >= 0
+    assert hit.sum() > n // 2 and np.count_nonzero(hit & (ref[:, 1] < 0)) > 10000
+
+
+def test_batch_workload_matches_single_view(workload, device):
+    """The batched-requests workload: 64 requests of sizes drawn from
+    {256, 512, 768, 1024}^2 with seeded orbit cameras and 4 random 5-8 point
+    TFs, 16 spp, one persistent launch; four of the requests (the first of
+    each size) against the single-view renderer with the same settings, whose
+    frames the tests above check against the oracle."""
+    import torch
+    from owlexastitcher_amd import RenderBatch, tf_rasterize
+    r, _, _, _ = workload
+    print("case: batch workload, 64 requests, 16 spp, 1 frame", flush=True)
+    rng = np.random.default_rng(1)
+    vr = r.valueRange
+    tfs = []
+    for _ in range(4):
+        npt = int(rng.integers(5, 9))
+        pts = np.zeros((npt, 5), np.float32)
+        pts[:, 0] = np.concatenate([[0.0], np.sort(rng.uniform(0.0, 1.0, npt - 2)), [1.0]])
+        pts[:, 1:4] = rng.uniform(0.0, 1.0, (npt, 3))
+        pts[:, 4] = rng.uniform(0.0, 1.0, npt)
+        # the opacity multiplier U[0.5, 4] as a slider value, os = 1.1^(s - 100)
+        slider = 100.0 + float(np.log(rng.uniform(0.5, 4.0)) / np.log(1.1))
+        r.setOpacityScale(slider)
+        tfs.append((tf_rasterize(pts, 128), slider, r.state().opacityScale))
+    bnd = r.modelBounds
+    c = 0.5 * (bnd[:3] + bnd[3:])
+    rad = 1.2 * float(np.linalg.norm(bnd[3:] - bnd[:3]))
+    reqs = []
+    for _ in range(64):
+        size = int(rng.choice([256, 512, 768, 1024]))
+        az, el = float(rng.uniform(0, 2 * np.pi)), float(rng.uniform(-0.8, 0.8))
+        eye = c + rad * np.array([np.cos(el) * np.cos(az), np.sin(el), np.cos(el) * np.sin(az)])
+        reqs.append((size, int(rng.integers(0, 4)), camera_look_at(eye, c, (0, 1, 0), 45.0, size, size)))
+    r.setRange(vr[0], vr[1])
+    r.setRelDomain(0.0, 100.0)
+    bt = RenderBatch(r)
+    ti = [bt.addTransferFunction(rgba, vr[0], vr[1], os) for rgba, _, os in tfs]
+    for size, tf, cam in reqs:
+        bt.addRequest(size, size, 16, ti[tf], cam, 0)
+    bt.resetAccum()
+    bt.render()
+    torch.cuda.synchronize()
+    picks = [next(i for i, q in enumerate(reqs) if q[0] == s) for s in (256, 512, 768, 1024)]
+    r.setScheme(RENDER_PERSISTENT)
+    r.setSpp(16)
+    r.setType(0)
+    for i in picks:
+        size, tf, cam = reqs[i]
+        r.resize(size, size)
+        r.setCamera(*cam)
+        r.setColorMap(tfs[tf][0])
+        r.setRange(vr[0], vr[1])
+        r.setOpacityScale(tfs[tf][1])
+        r.resetAccum()
+        fb = torch.zeros((size, size), dtype=torch.int32, device=device)
+        r.render(fb)
+        torch.cuda.synchronize()
+        acc_s, acc_b = r.accum(), bt.accum(i)
+        assert np.array_equal(acc_b, acc_s), f"request {i}: {np.count_nonzero(np.any(acc_b != acc_s, -1))} px differ"
+        assert np.array_equal(bt.frame(i), fb.cpu().numpy().view(np.uint32))
+        assert not np.all(acc_b == acc_b[:1, :1])
