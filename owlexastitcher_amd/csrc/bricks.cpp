@@ -48,6 +48,10 @@ void checkBrick(const Brick &b) {
     const int64_t hi = (int64_t)b.lower[k] + ((int64_t)b.size[k] << b.level);
     if (hi > INT32_MAX) throw std::runtime_error("brick extends beyond the int32 coordinate range");
   }
+  // cell counts are 32-bit offsets (begin); reject before the product can overflow
+  const int64_t xy = (int64_t)b.size[0] * b.size[1];
+  if (xy > (int64_t)UINT32_MAX || xy * b.size[2] > (int64_t)UINT32_MAX)
+    throw std::runtime_error("brick holds more than 2^32 cells");
 }
 
 // A brick's domain clipped to a region (ABRs.cpp buildPrims)
