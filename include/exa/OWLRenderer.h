@@ -6,6 +6,7 @@
  *   exa::ExaStitchModel    model/ExaStitchModel.h, ExaStitchModel.cpp:24-257
  *   exa::ExaBrickModel     model/ExaBrickModel.h, ExaBrickModel.cpp:27-213
  *   exa::ExaStitchSampler  sampler/Sampler.h:28-94, sampler/ExaStitchSampler.h:108-141
+ *   exa::ExaBrickSampler   sampler/ExaBrickSampler.h:98-241, ExaBrickSampler.cpp:26-247
  *   exa::ExaBrick          model/ABRs.h:28-63
  *
  * Header only; link with libexastitch.so.  Failures throw std::runtime_error
@@ -193,6 +194,36 @@ struct ExaStitchSampler {
     check(exa_sampler_locate(handle, d_pos, n, d_out, stream));
   }
   static std::string className() { return "ExaStitchSampler"; }
+};
+
+/* sampler/ExaBrickSampler in its default configuration (ABR traversal, ABR
+   sampling; ExaBrickSampler.cpp:26-247): build() makes the active brick
+   regions' device tables, computeMaxOpacities the per-region majorants
+   (regions at 0 are culled), sample() the region blend by point. */
+struct ExaBrickSampler {
+  typedef std::shared_ptr<ExaBrickSampler> SP;
+  exa_brick_sampler *handle = nullptr;
+  ExaBrickSampler() = default;
+  ~ExaBrickSampler() { exa_brick_sampler_destroy(handle); }
+  ExaBrickSampler(const ExaBrickSampler &) = delete;
+  ExaBrickSampler &operator=(const ExaBrickSampler &) = delete;
+  bool build(Model::SP model) {
+    auto m = std::dynamic_pointer_cast<ExaBrickModel>(model);
+    if (!m) return false;
+    exa_brick_sampler_destroy(handle);
+    handle = nullptr;
+    check(exa_brick_sampler_create(m->handle, nullptr, &handle));
+    return true;
+  }
+  void computeMaxOpacities(const std::vector<vec4f> &colorMap, range1f xfRange) {
+    check(exa_brick_sampler_compute_max_opacities(handle, &colorMap[0].x, (int)colorMap.size(), xfRange.lower,
+                                                  xfRange.upper, nullptr));
+  }
+  // sample() by point (ExaBrickSampler.h:221-236) over n device-resident points
+  void sample(const float *d_pos, int64_t n, exa_sample *d_out, void *stream = nullptr) const {
+    check(exa_brick_sampler_locate(handle, d_pos, n, d_out, stream));
+  }
+  static std::string className() { return "ExaBrickSampler"; }
 };
 
 struct OWLRenderer {

@@ -43,6 +43,10 @@ int main(int argc, char **argv) {
   b[0].size = {1, 1, 1};
   float v = 1.f;
   OWLRenderer rb(b.data(), &v, 1);
+  ExaBrickSampler bs;
+  auto bm = ExaBrickModel::load(b.data(), &v, 1);
+  std::printf("%d %d\n", (int)bs.build(bm), (int)bs.build(m));
+  bs.computeMaxOpacities({{0, 0, 0, 1}, {1, 1, 1, 1}}, bm->valueRange);
   return 0;
 }
 '''
