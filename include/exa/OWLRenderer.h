@@ -186,6 +186,7 @@ struct ExaStitchSampler {
     return true;
   }
   void computeMaxOpacities(const std::vector<vec4f> &colorMap, range1f xfRange) {
+    if (colorMap.empty()) throw std::runtime_error("empty colour map");
     check(exa_sampler_compute_max_opacities(handle, &colorMap[0].x, (int)colorMap.size(), xfRange.lower,
                                             xfRange.upper, nullptr));
   }
@@ -216,11 +217,14 @@ struct ExaBrickSampler {
     return true;
   }
   void computeMaxOpacities(const std::vector<vec4f> &colorMap, range1f xfRange) {
+    if (colorMap.empty()) throw std::runtime_error("empty colour map");
+    if (!handle) throw std::runtime_error("sampler not built");
     check(exa_brick_sampler_compute_max_opacities(handle, &colorMap[0].x, (int)colorMap.size(), xfRange.lower,
                                                   xfRange.upper, nullptr));
   }
   // sample() by point (ExaBrickSampler.h:221-236) over n device-resident points
   void sample(const float *d_pos, int64_t n, exa_sample *d_out, void *stream = nullptr) const {
+    if (!handle) throw std::runtime_error("sampler not built");
     check(exa_brick_sampler_locate(handle, d_pos, n, d_out, stream));
   }
   static std::string className() { return "ExaBrickSampler"; }
