@@ -460,8 +460,12 @@ void exa_brick_sampler_destroy(exa_brick_sampler *s);
    ignored); the ray marcher is the ExaBricks integrateDVR (deviceCode.cu:
    401-499).  Render counters: [0] samples blended, [1] brick records read,
    [2] kd leaves visited, [7] regions entered, [8] pixels, [9] TF fetches.
-   Accelerator, scheme, batch and neural entry points return EXA_ERR_STATE
-   (scheme and tuning are ignored). */
+   The scheme selects the launch as for exaStitch: EXA_RENDER_PERSISTENT
+   (default) runs the path tracer and direct lighting in persistent lanes
+   that take pixels from a work counter, EXA_RENDER_MEGAKERNEL one thread per
+   pixel; the ray marcher and heat-map frames always run one thread per
+   pixel; the frames are the same.  The persistent tuning is ignored;
+   accelerator, batch and neural entry points return EXA_ERR_STATE. */
 int exa_renderer_create_bricks(const exa_brick_model *m, const int32_t numMCs[3], const float remapFrom[6],
                                const float remapTo[6], exa_renderer **out);
 /* the renderer's brick sampler, or NULL for an exaStitch renderer */

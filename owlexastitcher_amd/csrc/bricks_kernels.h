@@ -24,10 +24,14 @@ struct BrickView {
 // point location: out[i] = {0, -1, value} inside a region that is not
 // culled, else {0, 0, 0}
 void launchBrickLocate(const BrickView &v, const float *pos, int64_t n, Sample *out, hipStream_t s);
-// one frame of the ExaBricks renderer (shade mode ignored: Default).
-// Counters: kCntGridletHit samples blended, kCntTetHit brick records read,
-// kCntPyrHit kd leaves visited, kCntDdaCells regions entered, kCntTfFetch,
-// kCntPixels.
-void launchRenderBricks(const RenderParams &p, const BrickView &v, unsigned long long *counters, hipStream_t s);
+// one frame of the ExaBricks renderer (shade mode ignored: Default).  With
+// `work` (one uint32 of device memory, reset by the launch) the path tracer
+// and direct lighting run in persistent lanes; the ray marcher, heat-map
+// frames and work == nullptr run one thread per pixel; both give the same
+// frame.  Counters: kCntGridletHit samples blended, kCntTetHit brick records
+// read, kCntPyrHit kd leaves visited, kCntDdaCells regions entered,
+// kCntTfFetch, kCntPixels.
+void launchRenderBricks(const RenderParams &p, const BrickView &v, uint32_t *work, unsigned long long *counters,
+                        hipStream_t s);
 
 }  // namespace exa

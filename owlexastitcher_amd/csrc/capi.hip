@@ -1132,7 +1132,12 @@ static void doRender(exa_renderer *r, uint32_t *fb, unsigned long long *counters
   P.tfN = (int32_t)r->tf.count;
   if (r->bricks) {
     if (!r->bricks->haveMaxOp) throw Status(EXA_ERR_STATE, "region max opacities not computed");
-    launchRenderBricks(P, r->bricks->view(), counters, s);
+    uint32_t *work = nullptr;
+    if (r->scheme == EXA_RENDER_PERSISTENT) {
+      r->work.alloc(1);
+      work = r->work.ptr;
+    }
+    launchRenderBricks(P, r->bricks->view(), work, counters, s);
     HIP_CHECK(hipGetLastError());
     r->accumID++;
     return;

@@ -4,7 +4,10 @@
 // in its default configuration (EXABRICK_ABR_TRAVERSAL with the ABR sampling
 // BVH): majorants per region, the regions a ray crosses walked front to back
 // on the kd-tree, the sample at a tentative collision blended over the
-// current region's bricks.  One thread per pixel, 16x16 pixel tiles.
+// current region's bricks.  One thread per pixel in 16x16 pixel tiles, or
+// (path tracer and direct lighting) persistent lanes that take pixels from a
+// work counter.
+#include <algorithm>
 #include "bricks_device.h"
 #include "exa_integrate.h"
 
@@ -205,15 +208,193 @@ __global__ __launch_bounds__(kRenderBlock) void kRenderBricks(RenderParams P, Br
   }
 }
 
+// Persistent variant for the path tracer and direct lighting: resident
+// waves whose lanes take pixels (8x8 tiles in scan order) from a frame-wide
+// counter, one atomic per wave, and run their paths as a state machine whose
+// one hot step is a sampleInteraction call, the main ray's or the light
+// sample's.  A lane whose path has ended takes the next sample or pixel
+// while the others still walk regions, so a wave no longer waits for its
+// longest path.  Each lane draws its random numbers and does its arithmetic
+// in the order of pathTrace / directLight above, so the frames equal the
+// one-thread-per-pixel kernel's bit for bit.
+constexpr int kBrickPersistBlock = 256;
+constexpr int kBrickPixelTile = 8;
+enum : int { kBrGrab = 0, kBrStart = 1, kBrMain = 2, kBrShadow = 3, kBrDone = 4 };
+
+__global__ void kResetBrickWork(uint32_t *work) {
+  if (threadIdx.x == 0) *work = 0u;
+}
+
+template <bool kCount>
+__global__ __launch_bounds__(kBrickPersistBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) void kRenderBricksPersistent(RenderParams P, BrickView bv,
+                                                                              uint32_t *work, uint32_t numWork,
+                                                                              unsigned long long *counters) {
+  const int lane = threadIdx.x & 63;
+  const int tilesX = (P.W + kBrickPixelTile - 1) / kBrickPixelTile;
+  const int numLights = countLights(P);
+  const bool direct = P.integrator == 1;
+  LocalCounters cnt;
+  if (kCount)
+    for (int i = 0; i < kNumCounters; ++i) cnt.c[i] = 0;
+  Lcg rng;
+  BrickCtx<kCount> C{P, bv, rng, cnt, 0, 0};
+  int st = kBrGrab, s = 0;
+  float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+  // path state of pathTrace / directLight; the light sample's ray starts at
+  // the scattering point (qray.org), and directLight keeps its albedo in
+  // throughput (still 1 there, so 1 * (albedo * L) and albedo * L agree)
+  Ray ray, qray;
+  vec3f throughput = mk3s(1.f), Ld = mk3s(0.f);
+  unsigned bounce = 0;
+  float t0 = 1e30f, t1 = -1e30f, ld = 0.f;
+  int which = 0;
+  auto endSample = [&](float4 c) {
+    acc.x += c.x;
+    acc.y += c.y;
+    acc.z += c.z;
+    acc.w += c.w;
+    if (++s < P.spp) {
+      st = kBrStart;
+      return;
+    }
+    finishPixel(P, C.px, C.py, acc);
+    if (kCount) cnt.c[kCntPixels]++;
+    st = kBrGrab;
+  };
+  auto scatter = [&](vec3f pos) {
+    const vec3f scatterDir = hgSample(-ray.dir, rng);
+    ray.org = pos;
+    ray.dir = scatterDir;
+    boxTest(ray, P.worldBounds, t0, t1);
+    clipInterval(P, ray, t0, t1);
+    ray.tmax = t1;
+    st = kBrMain;
+  };
+  for (;;) {
+    const uint64_t want = __ballot(st == kBrGrab);
+    if (want) {
+      const int leader = __ffsll((unsigned long long)want) - 1;
+      uint32_t base = 0;
+      if (lane == leader) base = atomicAdd(work, (uint32_t)__popcll(want));
+      base = __shfl(base, leader);
+      if (st == kBrGrab) {
+        const uint32_t w = base + (uint32_t)__popcll(want & ((1ull << lane) - 1ull));
+        if (w >= numWork) {
+          st = kBrDone;
+        } else {
+          const uint32_t tile = w >> 6, in = w & 63u;
+          const int px = (int)(tile % (uint32_t)tilesX) * kBrickPixelTile + (int)(in & 7u);
+          const int py = (int)(tile / (uint32_t)tilesX) * kBrickPixelTile + (int)(in >> 3);
+          if (px < P.W && py < P.H) {  // else outside a partial edge tile: grab again
+            C.px = px;
+            C.py = py;
+            rng.init((uint32_t)(px + P.W * py), (uint32_t)P.accumID);
+            acc = make_float4(0.f, 0.f, 0.f, 0.f);
+            s = 0;
+            st = kBrStart;
+          }
+        }
+      }
+    }
+    if (__ballot(st != kBrDone) == 0) break;
+    if (st == kBrStart) {
+      ray = primaryRay(P, C.px, C.py, rng);
+      throughput = mk3s(1.f);
+      Ld = mk3s(0.f);
+      bounce = 0;
+      t0 = 1e30f;
+      t1 = -1e30f;
+      if (boxTest(ray, P.worldBounds, t0, t1)) {
+        clipInterval(P, ray, t0, t1);
+        ray.org = ray.org + ray.dir * t0;
+        t1 -= t0;
+        ray.tmax = t1;
+        st = kBrMain;
+      } else {
+        const float4 bg = backgroundColor(C.py, P.H);
+        endSample(direct ? bg : ptResult(Ld, throughput, bounce, numLights, bg));
+      }
+    }
+    if (st == kBrMain || st == kBrShadow) {
+      const bool shadow = st == kBrShadow;
+      const Interaction I = sampleInteraction(C, shadow ? qray : ray, shadow);
+      if (direct) {
+        if (shadow) {
+          throughput = throughput * lightResult(P, which, I.Tr, ld);
+          endSample(make_float4(throughput.x, throughput.y, throughput.z, 1.f));
+        } else if (I.type == kBoundary) {
+          endSample(backgroundColor(C.py, P.H));
+        } else {
+          throughput = mk3(I.xf.x, I.xf.y, I.xf.z);
+          if (numLights > 0) {
+            qray = lightRay(P, I.pos, numLights, rng, which, ld);
+            st = kBrShadow;
+          } else {
+            endSample(make_float4(throughput.x, throughput.y, throughput.z, 1.f));
+          }
+        }
+      } else if (shadow) {
+        Ld = Ld + throughput * lightResult(P, which, I.Tr, ld);
+        scatter(qray.org);
+      } else if (I.type == kBoundary || !ptInteraction(P, I.xf, throughput, bounce, rng)) {
+        endSample(ptResult(Ld, throughput, bounce, numLights, backgroundColor(C.py, P.H)));
+      } else if (numLights > 0) {
+        qray = lightRay(P, I.pos, numLights, rng, which, ld);
+        st = kBrShadow;
+      } else {
+        scatter(I.pos);
+      }
+    }
+  }
+  if (kCount)
+    for (int i = 0; i < kNumCounters; ++i)
+      if (cnt.c[i]) atomicAdd(&counters[i], (unsigned long long)cnt.c[i]);
+}
+
+template <bool kCount>
+static int residentBrickBlocks() {
+  static int cached = 0;
+  if (!cached) {
+    int perCU = 0, dev = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, kRenderBricksPersistent<kCount>, kBrickPersistBlock,
+                                                     0) != hipSuccess ||
+        perCU < 1)
+      perCU = 1;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+      cus = 256;
+    cached = perCU * cus;
+  }
+  return cached;
+}
+
 void launchBrickLocate(const BrickView &v, const float *pos, int64_t n, Sample *out, hipStream_t s) {
   if (n <= 0) return;
   const int64_t blocks = (n + 255) / 256;
   hipLaunchKernelGGL(kBrickLocate, dim3((unsigned)blocks), dim3(256), 0, s, v, pos, n, out);
 }
 
-void launchRenderBricks(const RenderParams &p, const BrickView &v, unsigned long long *counters, hipStream_t s) {
+void launchRenderBricks(const RenderParams &p, const BrickView &v, uint32_t *work, unsigned long long *counters,
+                        hipStream_t s) {
   RenderParams q = p;
   q.shadeMode = 0;  // renderFrame_ExaBrickSampler shades with Default only
+  if (work && q.integrator != 2 && !q.heatMapEnabled) {
+    const uint32_t tilesX = (uint32_t)((q.W + kBrickPixelTile - 1) / kBrickPixelTile);
+    const uint32_t tilesY = (uint32_t)((q.H + kBrickPixelTile - 1) / kBrickPixelTile);
+    const uint32_t numWork = tilesX * tilesY * 64u;
+    const int want = (int)((numWork + kBrickPersistBlock - 1) / kBrickPersistBlock);
+    hipLaunchKernelGGL(kResetBrickWork, dim3(1), dim3(64), 0, s, work);
+    if (counters) {
+      const int blocks = std::max(1, std::min(residentBrickBlocks<true>(), want));
+      hipLaunchKernelGGL((kRenderBricksPersistent<true>), dim3(blocks), dim3(kBrickPersistBlock), 0, s, q, v, work,
+                         numWork, counters);
+    } else {
+      const int blocks = std::max(1, std::min(residentBrickBlocks<false>(), want));
+      hipLaunchKernelGGL((kRenderBricksPersistent<false>), dim3(blocks), dim3(kBrickPersistBlock), 0, s, q, v, work,
+                         numWork, counters);
+    }
+    return;
+  }
   const dim3 grid((q.W + kTileX - 1) / kTileX, (q.H + kTileY - 1) / kTileY);
   const dim3 block(kTileX, kTileY);
   if (counters) hipLaunchKernelGGL((kRenderBricks<true>), grid, block, 0, s, q, v, counters);

@@ -196,3 +196,36 @@ def test_render_clip_remap_spp(device, integrator):
     assert np.array_equal(acc, acc_o), _report(acc, acc_o)
     assert np.array_equal(fb, fb_o)
     assert not np.all(acc == acc[:, :1])
+
+
+@pytest.mark.parametrize("integrator", [0, 1])
+def test_persistent_matches_megakernel(device, integrator):
+    """The persistent brick renderer (lanes refilled from a pixel counter) and
+    the one-thread-per-pixel kernel give the same frames and counters; a
+    size with partial edge tiles, 2 spp, a point and a directional light."""
+    import torch
+    from owlexastitcher_amd import RENDER_MEGAKERNEL, RENDER_PERSISTENT
+    print(f"case: brick persistent vs megakernel, integrator={integrator}, 83x61, 2 spp", flush=True)
+    model = ExaBrickModel.from_amr(_scene("noise"), 8)
+    r = make_renderer(model, 83, 61)
+    r.setType(integrator)
+    r.setMaxBounces(6)
+    r.setSpp(2)
+    r.setDirectionalLight(0, (0.3, 1.0, 0.5), 2.0, True)
+    r.setLightSource(1, r.modelBounds[3:] + 5.0, 4.0e4, True)
+    fb = torch.zeros((61, 83), dtype=torch.int32, device=device)
+    out = {}
+    for scheme in (RENDER_PERSISTENT, RENDER_MEGAKERNEL):
+        r.setScheme(scheme)
+        r.accumID = 0
+        r.render(fb)
+        c = r.render_counted(fb)
+        torch.cuda.synchronize()
+        out[scheme] = (r.accum(), fb.cpu().numpy().copy(), c)
+    a0, f0, c0 = out[RENDER_PERSISTENT]
+    a1, f1, c1 = out[RENDER_MEGAKERNEL]
+    assert np.array_equal(a0, a1), _report(a0, a1)
+    assert np.array_equal(f0, f1)
+    assert c0 == c1 and c0["pixels"] == 83 * 61
+    assert not np.all(a0 == a0[:, :1])
+
