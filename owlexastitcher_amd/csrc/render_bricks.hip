@@ -209,14 +209,16 @@ __global__ __launch_bounds__(kRenderBlock) void kRenderBricks(RenderParams P, Br
 }
 
 // Persistent variant for the path tracer and direct lighting: resident
-// waves whose lanes take pixels (8x8 tiles in scan order) from a frame-wide
-// counter, one atomic per wave, and run their paths as a state machine whose
-// one hot step is a sampleInteraction call, the main ray's or the light
-// sample's.  A lane whose path has ended takes the next sample or pixel
-// while the others still walk regions, so a wave no longer waits for its
-// longest path.  Each lane draws its random numbers and does its arithmetic
-// in the order of pathTrace / directLight above, so the frames equal the
-// one-thread-per-pixel kernel's bit for bit.
+// waves (4 per SIMD) whose lanes take pixels (8x8 tiles in scan order) from
+// a frame-wide counter, one atomic per wave, and run their paths as a state
+// machine whose hot step is one pass of sampleInteraction's region loop (the
+// next region of the main ray's or the light sample's query and the tracking
+// steps inside it).  A lane whose query has ended moves on to the next
+// segment, sample or pixel while the others still walk regions, so a wave
+// waits neither for its longest path nor for its longest region walk.  Each
+// lane draws its random numbers and does its arithmetic in the order of
+// sampleInteraction and pathTrace / directLight above, so the frames equal
+// the one-thread-per-pixel kernel's bit for bit.
 constexpr int kBrickPersistBlock = 256;
 constexpr int kBrickPixelTile = 8;
 enum : int { kBrGrab = 0, kBrStart = 1, kBrMain = 2, kBrShadow = 3, kBrDone = 4 };
@@ -248,6 +250,16 @@ __global__ __launch_bounds__(kBrickPersistBlock) __attribute__((amdgpu_waves_per
   unsigned bounce = 0;
   float t0 = 1e30f, t1 = -1e30f, ld = 0.f;
   int which = 0;
+  // the interaction query in progress (sampleInteraction's locals)
+  Interaction I;
+  I.type = kBoundary;
+  vec3f qo = mk3s(0.f), qd = mk3s(0.f);
+  float already = 0.f;
+  AbrWalk walk;
+  walkBegin(walk, 0.f, 0.f);
+  int visits = 0;
+  bool inQuery = false;
+  uint32_t unused = 0;
   auto endSample = [&](float4 c) {
     acc.x += c.x;
     acc.y += c.y;
@@ -317,7 +329,57 @@ __global__ __launch_bounds__(kBrickPersistBlock) __attribute__((amdgpu_waves_per
     }
     if (st == kBrMain || st == kBrShadow) {
       const bool shadow = st == kBrShadow;
-      const Interaction I = sampleInteraction(C, shadow ? qray : ray, shadow);
+      if (!inQuery) {
+        // sampleInteraction's set-up: the query ray in voxel space
+        const Ray &q = shadow ? qray : ray;
+        I.type = kBoundary;
+        I.Tr = 1.f;
+        I.pos = mk3s(0.f);
+        I.xf = make_float4(0.f, 0.f, 0.f, 0.f);
+        qo = xfmPoint(P.voxelXfm, q.org);
+        vec3f dir = xfmVector(P.voxelXfm, q.dir);
+        const float dt_scale = length(dir);
+        qd = normalize(dir);
+        already = q.tmin * dt_scale;
+        walkBegin(walk, already, q.tmax * dt_scale);
+        visits = 0;
+        inQuery = true;
+      }
+      // one pass of sampleInteraction's region loop: the next region and the
+      // tracking steps inside it
+      bool finished = visits >= kMaxAbrVisits;
+      if (!finished) {
+        ++visits;
+        float r0, r1;
+        const int32_t abr = walkNext(bv, walk, qo, qd, already, r0, r1, kCount ? cnt.c[kCntPyrHit] : unused);
+        if (abr < 0) {
+          finished = true;
+        } else {
+          if (kCount) cnt.c[kCntDdaCells]++;
+          const float majorant = bv.abrMaxOp[abr];
+          float t = r0;
+          int32_t primID = -1;
+          for (;;) {
+            if (majorant <= 0.f) break;
+            if (!trackingStep(t, majorant, P.opacityScale, r1, rng)) break;
+            I.pos = qo + qd * t;
+            const Sample smp{0, -1, abrValue(bv, abr, I.pos, kCount ? cnt.c[kCntTetHit] : unused)};
+            if (kCount) {
+              cnt.c[kCntGridletHit]++;
+              cnt.c[kCntTfFetch]++;
+            }
+            if (resolveCollision(P, nullptr, C.px, C.py, smp, shadow, majorant, rng, primID, I.xf, I.Tr)) {
+              I.type = kScattering;
+              finished = true;
+              break;
+            }
+          }
+          if (!finished) already = r1 * 1.0000001f;
+        }
+      }
+      if (!finished) continue;
+      inQuery = false;
+      I.pos = xfmPoint(P.voxelXfmInv, I.pos);
       if (direct) {
         if (shadow) {
           throughput = throughput * lightResult(P, which, I.Tr, ld);
