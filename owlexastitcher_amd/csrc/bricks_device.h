@@ -127,6 +127,54 @@ EXA_DEV void walkBegin(AbrWalk &w, float tmin, float tmax) {
   w.dropped = false;
 }
 
+// One plane of a descent at node n toward the leaf holding ta: pushes the far
+// child with the interval's end when the far side is still inside it (the
+// stack's top is entry 0, static register moves) and returns the child to
+// descend into.
+EXA_DEV int32_t walkDescend(AbrWalk &w, int2 n, int32_t node, vec3f org, vec3f dir, float ta, float &tb) {
+  const int dim = n.y & 3;
+  const float d = comp(dir, dim);
+  const float ts = (__int_as_float(n.x) - comp(org, dim)) / d;
+  const int32_t lowerChild = n.y >> 2, upperChild = node + 1;
+  const bool neg = signbit(d);
+  const int32_t nearC = neg ? upperChild : lowerChild, farC = neg ? lowerChild : upperChild;
+  if (!(ta < ts)) return farC;
+  if (ts < tb) {
+    if (w.sp == kWalkStack) w.dropped = true;
+    else ++w.sp;
+#pragma unroll
+    for (int k = kWalkStack - 1; k > 0; --k) {
+      w.sn[k] = w.sn[k - 1];
+      w.st[k] = w.st[k - 1];
+    }
+    w.sn[0] = farC;
+    w.st[0] = tb;
+    tb = ts;
+  }
+  return nearC;
+}
+
+// The continuation after a leaf whose kd interval ends at tb: the stack's top,
+// or the root (an empty stack with nothing dropped leaves tb == tmax, which
+// ends the walk).
+EXA_DEV void walkLeafDone(AbrWalk &w, float tb) {
+  w.ta = tb;
+  if (w.sp > 0) {
+    w.node = w.sn[0];
+    w.tb = w.st[0];
+#pragma unroll
+    for (int k = 0; k < kWalkStack - 1; ++k) {
+      w.sn[k] = w.sn[k + 1];
+      w.st[k] = w.st[k + 1];
+    }
+    --w.sp;
+  } else {
+    w.node = 0;
+    w.tb = w.tmax;
+    w.dropped = false;
+  }
+}
+
 // The first region not culled whose box test from tq passes, or -1.
 EXA_DEV int32_t walkNext(const BrickView &v, AbrWalk &w, vec3f org, vec3f dir, float tq, float &t0, float &t1,
                          uint32_t &leaves) {
@@ -136,52 +184,47 @@ EXA_DEV int32_t walkNext(const BrickView &v, AbrWalk &w, vec3f org, vec3f dir, f
     const float ta = w.ta;
     int2 n = v.nodes[node];
     while ((n.y & 3) != 3) {
-      const int dim = n.y & 3;
-      const float d = comp(dir, dim);
-      const float ts = (__int_as_float(n.x) - comp(org, dim)) / d;
-      const int32_t lowerChild = n.y >> 2, upperChild = node + 1;
-      const bool neg = signbit(d);
-      const int32_t nearC = neg ? upperChild : lowerChild, farC = neg ? lowerChild : upperChild;
-      if (ta < ts) {
-        if (ts < tb) {
-          // push (farC, tb); the stack's top is entry 0 (static register moves)
-          if (w.sp == kWalkStack) w.dropped = true;
-          else ++w.sp;
-#pragma unroll
-          for (int k = kWalkStack - 1; k > 0; --k) {
-            w.sn[k] = w.sn[k - 1];
-            w.st[k] = w.st[k - 1];
-          }
-          w.sn[0] = farC;
-          w.st[0] = tb;
-          tb = ts;
-        }
-        node = nearC;
-      } else {
-        node = farC;
-      }
+      node = walkDescend(w, n, node, org, dir, ta, tb);
       n = v.nodes[node];
     }
     ++leaves;
-    // the continuation after this leaf, whose kd interval ends at tb
-    w.ta = tb;
-    if (w.sp > 0) {
-      w.node = w.sn[0];
-      w.tb = w.st[0];
-#pragma unroll
-      for (int k = 0; k < kWalkStack - 1; ++k) {
-        w.sn[k] = w.sn[k + 1];
-        w.st[k] = w.st[k + 1];
-      }
-      --w.sp;
-    } else {
-      // an empty stack with nothing dropped leaves tb == tmax: the loop ends
-      w.node = 0;
-      w.tb = w.tmax;
-      w.dropped = false;
-    }
+    walkLeafDone(w, tb);
     const int32_t abr = (n.y >> 2) - 1;
     if (abr >= 0 && !abrCulled(v, abr) && abrBoxTest(v, abr, org, dir, tq, w.tmax, t0, t1)) return abr;
+  }
+  return -1;
+}
+
+// walkNext in bounded pieces: at most `budget` kd nodes (planes descended and
+// leaves reached) per call.  Returns kWalkPending when the budget runs out
+// before a region is found or the walk ends, with the walk's place kept in w
+// (the descent's node and interval end) so that the next call continues it;
+// `guard` counts the leaves reached since the last completed call (the
+// caller resets it to 0 after a result other than kWalkPending).  A sequence
+// of calls does exactly the work of one walkNext and returns its result.
+constexpr int32_t kWalkPending = -2;
+EXA_DEV int32_t walkNextBounded(const BrickView &v, AbrWalk &w, vec3f org, vec3f dir, float tq, float &t0,
+                                float &t1, uint32_t &leaves, int budget, int32_t &guard) {
+  while (guard < kMaxAbrVisits && w.ta < w.tmax) {
+    int32_t node = w.node;
+    float tb = w.tb;
+    const float ta = w.ta;
+    int2 n = v.nodes[node];
+    while ((n.y & 3) != 3) {
+      if (budget-- <= 0) {
+        w.node = node;
+        w.tb = tb;
+        return kWalkPending;
+      }
+      node = walkDescend(w, n, node, org, dir, ta, tb);
+      n = v.nodes[node];
+    }
+    ++guard;
+    ++leaves;
+    walkLeafDone(w, tb);
+    const int32_t abr = (n.y >> 2) - 1;
+    if (abr >= 0 && !abrCulled(v, abr) && abrBoxTest(v, abr, org, dir, tq, w.tmax, t0, t1)) return abr;
+    if (budget-- <= 0) return kWalkPending;
   }
   return -1;
 }

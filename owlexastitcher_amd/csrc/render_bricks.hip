@@ -213,13 +213,16 @@ __global__ __launch_bounds__(kRenderBlock) void kRenderBricks(RenderParams P, Br
 // a frame-wide counter, one atomic per wave, and run their paths as a state
 // machine whose hot step is one pass of sampleInteraction's region loop (the
 // next region of the main ray's or the light sample's query and the tracking
-// steps inside it).  A lane whose query has ended moves on to the next
+// steps inside it), the region walk cut into pieces of kBrickWalkBudget kd
+// nodes.  A lane whose query has ended moves on to the next
 // segment, sample or pixel while the others still walk regions, so a wave
 // waits neither for its longest path nor for its longest region walk.  Each
 // lane draws its random numbers and does its arithmetic in the order of
 // sampleInteraction and pathTrace / directLight above, so the frames equal
 // the one-thread-per-pixel kernel's bit for bit.
 constexpr int kBrickPersistBlock = 256;
+// kd nodes a lane walks per step before it yields to the lanes in other states
+constexpr int kBrickWalkBudget = 8;
 constexpr int kBrickPixelTile = 8;
 enum : int { kBrGrab = 0, kBrStart = 1, kBrMain = 2, kBrShadow = 3, kBrDone = 4 };
 
@@ -257,7 +260,7 @@ __global__ __launch_bounds__(kBrickPersistBlock) __attribute__((amdgpu_waves_per
   float already = 0.f;
   AbrWalk walk;
   walkBegin(walk, 0.f, 0.f);
-  int visits = 0;
+  int visits = 0, guard = 0;
   bool inQuery = false;
   uint32_t unused = 0;
   auto endSample = [&](float4 c) {
@@ -343,16 +346,23 @@ __global__ __launch_bounds__(kBrickPersistBlock) __attribute__((amdgpu_waves_per
         already = q.tmin * dt_scale;
         walkBegin(walk, already, q.tmax * dt_scale);
         visits = 0;
+        guard = 0;
         inQuery = true;
       }
       // one pass of sampleInteraction's region loop: the next region and the
       // tracking steps inside it
       bool finished = visits >= kMaxAbrVisits;
       if (!finished) {
-        ++visits;
         float r0, r1;
-        const int32_t abr = walkNext(bv, walk, qo, qd, already, r0, r1, kCount ? cnt.c[kCntPyrHit] : unused);
-        if (abr < 0) {
+        const int32_t abr = walkNextBounded(bv, walk, qo, qd, already, r0, r1, kCount ? cnt.c[kCntPyrHit] : unused,
+                                            kBrickWalkBudget, guard);
+        if (abr != kWalkPending) {
+          ++visits;
+          guard = 0;
+        }
+        if (abr == kWalkPending) {
+          // the walk continues in the next step
+        } else if (abr < 0) {
           finished = true;
         } else {
           if (kCount) cnt.c[kCntDdaCells]++;
