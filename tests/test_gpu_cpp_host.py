@@ -42,6 +42,9 @@ CASES = {
                                      "-sm", "1", "--clip-plane", "0", "0", "1", "30", "-spp", "2"],
     "bricks_xf": lambda p: ["-bricks", p["bricks"], "-scalars", p["scalars"], "-xf",
                             os.path.join(GOLDEN, "cloud.xf")],
+    "subimg_range_remap": lambda p: [p["exum"], "-grids", p["grids"], "-scalars", p["scalars"], "--subimg", "0.2",
+                                     "0.1", "0.7", "0.8", "--range", "0.1", "0.9", "-spp", "3", "--remap-from", "0",
+                                     "0", "0", "64", "64", "64", "--remap-to", "0", "0", "0", "32", "32", "32"],
 }
 
 
