@@ -7,7 +7,7 @@ directional light, the 5-point TF at opacity scale 100.  The majorant grid and
 the per-primitive culling come from the oracle's own build and are checked
 against the library's; then two frames of each launch scheme are compared with
 the oracle's accumulation buffer and RGBA8 frame buffer, every pixel.  The
-locate and batched-requests workloads follow at their full sizes.
+locate, batched-requests and ExaBricks workloads follow at their full sizes.
 """
 import numpy as np
 import pytest
@@ -23,9 +23,13 @@ W, H = 1920, 1080
 
 
 @pytest.fixture(scope="module")
-def workload():
+def amr():
+    return AmrScene.noise(128, 2, 0.55, 0.7, 6, 2.0, 1)
+
+
+@pytest.fixture(scope="module")
+def workload(amr):
     print("case: timed workload build (noise 128^3 base, 1080p)", flush=True)
-    amr = AmrScene.noise(128, 2, 0.55, 0.7, 6, 2.0, 1)
     dm = DualMesh(amr)
     model = ExaStitchModel.from_dual_mesh(dm)
     r = OWLRenderer(model, (64, 64, 64))
@@ -165,3 +169,56 @@ def test_batch_workload_matches_single_view(workload, device):
         assert np.array_equal(acc_b, acc_s), f"request {i}: {np.count_nonzero(np.any(acc_b != acc_s, -1))} px differ"
         assert np.array_equal(bt.frame(i), fb.cpu().numpy().view(np.uint32))
         assert not np.all(acc_b == acc_b[:1, :1])
+
+
+def test_bricks_workload_full_size(amr, workload, device):
+    """The ExaBricks workload at its full size (the same leaf cells as 503K
+    bricks of the brick maker, 23.6M regions): point location at 4096 random
+    points against the brute-force blend over every brick (hits exactly where
+    some brick's basis functions have weight, values bit-identical), and the
+    1080p frame of the persistent kernel against the one-thread-per-pixel
+    kernel, bit for bit."""
+    import torch
+    from owlexastitcher_amd import ExaBrickModel, ExaBrickSampler
+    print("case: bricks workload full size, locate 4096 points + 1080p frame, both schemes", flush=True)
+    r, _, _, rgba = workload
+    bm = ExaBrickModel.from_amr(amr, 16)
+    assert bm.counts()["abrs"] > 20_000_000
+    bricks, scalars, _, _ = bm.arrays()
+    s = ExaBrickSampler(bm)
+    cb, _ = bm.bounds()
+    p = np.random.default_rng(3).uniform(cb[:3] - 4, cb[3:] + 4, (4096, 3)).astype(np.float32)
+    d_pos = torch.from_numpy(p).to(device)
+    d_out = torch.full((len(p), 3), 7, dtype=torch.int32, device=device)
+    s.locate(d_pos, d_out)
+    torch.cuda.synchronize()
+    out = d_out.cpu().numpy()
+    v, w = ora.brick_blend_brute(bricks, scalars, p)
+    hit = out[:, 1] == -1
+    assert np.array_equal(hit, w > 0) and hit.sum() > 1000
+    assert np.array_equal(out[hit, 2].view(np.float32), v[hit])
+    rb = OWLRenderer(bm, (64, 64, 64))
+    st = r.state()
+    rb.resize(W, H)
+    rb.setCamera(st.camOrg, st.camDir00, st.camDu, st.camDv)
+    rb.setColorMap(rgba)
+    vr = rb.valueRange
+    rb.setRange(vr[0], vr[1])
+    rb.setRelDomain(0.0, 100.0)
+    rb.setOpacityScale(100.0)
+    rb.setType(0)
+    rb.setMaxBounces(4)
+    rb.setDirectionalLight(0, (0.3, 1.0, 0.5), 2.0, True)
+    fb = torch.zeros((H, W), dtype=torch.int32, device=device)
+    frames = []
+    for scheme in (RENDER_PERSISTENT, RENDER_MEGAKERNEL):
+        rb.setScheme(scheme)
+        rb.accumID = 0
+        rb.render(fb)
+        torch.cuda.synchronize()
+        frames.append((rb.accum(), fb.cpu().numpy().copy()))
+    (a0, f0), (a1, f1) = frames
+    assert np.count_nonzero(np.any(a0 != a1, axis=-1)) == 0
+    assert np.array_equal(f0, f1)
+    assert np.count_nonzero(np.any(a0 != a0[:1, :1], axis=-1)) > W * H // 10
+
