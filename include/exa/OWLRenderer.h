@@ -5,6 +5,7 @@
  *   exa::Model             model/Model.h:29-65
  *   exa::ExaStitchModel    model/ExaStitchModel.h, ExaStitchModel.cpp:24-257
  *   exa::ExaBrickModel     model/ExaBrickModel.h, ExaBrickModel.cpp:27-213
+ *   exa::Sampler           sampler/Sampler.h:27-69 (abstract plugin base)
  *   exa::ExaStitchSampler  sampler/Sampler.h:28-94, sampler/ExaStitchSampler.h:108-141
  *   exa::ExaBrickSampler   sampler/ExaBrickSampler.h:98-241, ExaBrickSampler.cpp:26-247
  *   exa::ExaBrick          model/ABRs.h:28-63
@@ -83,9 +84,13 @@ inline void boxArray(const box3f &b, float o[6]) {
    grid dims and the voxel-space remap are stored until a renderer or a
    sampler is built from the model; the remapped bounds (Model::getBounds,
    Model.cpp:50-70) are OWLRenderer::modelBounds, computed by the library. */
-struct Model {
+struct Model : std::enable_shared_from_this<Model> {
   typedef std::shared_ptr<Model> SP;
   virtual ~Model() = default;
+  template <typename T>
+  std::shared_ptr<T> as() {
+    return std::dynamic_pointer_cast<T>(shared_from_this());
+  }
   box3f cellBounds;
   range1f valueRange{1e30f, -1e30f};
   vec3i numGridCells{128, 128, 128};
@@ -161,21 +166,72 @@ struct ExaBrickModel : Model {
   }
 };
 
+struct OWLRenderer;
+
+/* sampler/Sampler.h:27-69: the plugin base a renderer dispatches on.  A
+   sampler is built from a model (false when the model is not of its kind),
+   maps its value ranges through a colour map into majorants, and answers
+   point queries (the device contract sample(), here over device-resident
+   points in bulk).  OWLRenderer picks the subclass by the model's type
+   (OWLRenderer.cpp:315-325) and lets it make the renderer whose launch reads
+   its device data, as "renderFrame_" + className() selects the raygen program
+   (OWLRenderer.cpp:334-337) and getLPVariables/setLPs pack the sampler's
+   launch parameters; a sampler made by a renderer answers for that
+   renderer's data until the renderer is destroyed. */
+struct Sampler : std::enable_shared_from_this<Sampler> {
+  typedef std::shared_ptr<Sampler> SP;
+  virtual ~Sampler() = default;
+  template <typename T>
+  std::shared_ptr<T> as() {
+    return std::dynamic_pointer_cast<T>(shared_from_this());
+  }
+  virtual bool build(Model::SP model) = 0;
+  virtual void computeMaxOpacities(const std::vector<vec4f> &colorMap, range1f xfRange) = 0;
+  //! simple reflection: the exact name of the subclass
+  virtual std::string className() = 0;
+  //! sample() over n device-resident points; exa_sample = Sample (Sampler.h:72-76)
+  virtual void sample(const float *d_pos, int64_t n, exa_sample *d_out, void *stream = nullptr) const = 0;
+  //! the majorants (max opacities), one per macrocell or brick region, on the host
+  virtual std::vector<float> maxOpacities() const = 0;
+
+  /*! the accel used for space skipping; after a successful build exactly
+      one of these is set (Sampler.h:55-62) */
+  struct Grid {
+    vec3i dims;
+    box3f bounds;
+  };
+  struct KDTree {
+    int64_t numNodes = 0, depth = 0;
+  };
+  struct {
+    std::shared_ptr<Grid> grid;
+    std::shared_ptr<KDTree> kdtree;
+  } majorantAccel;
+
+ protected:
+  friend struct OWLRenderer;
+  virtual exa_renderer *createRenderer(Model::SP model, const int32_t numMCs[3], const float from[6],
+                                       const float to[6]) = 0;
+  virtual void releaseRenderer() = 0;
+};
+
 /* sampler/ExaStitchSampler: build() makes the majorant grid, the
    primitive value ranges and the point-location structures;
    computeMaxOpacities maps them through a colour map; sample() answers
    point queries in bulk (device pointers, exa_sample records). */
-struct ExaStitchSampler {
+struct ExaStitchSampler : Sampler {
   typedef std::shared_ptr<ExaStitchSampler> SP;
-  exa_sampler *handle = nullptr;
-  ExaStitchSampler() { check(exa_sampler_create(&handle)); }
-  ~ExaStitchSampler() { exa_sampler_destroy(handle); }
+  exa_sampler *handle = nullptr;  // built by build(); owned
+  ExaStitchSampler() = default;
+  ~ExaStitchSampler() override { exa_sampler_destroy(handle); }
   ExaStitchSampler(const ExaStitchSampler &) = delete;
   ExaStitchSampler &operator=(const ExaStitchSampler &) = delete;
   // Sampler::build (Sampler.h:44): false when the model is not an exaStitch model
-  bool build(Model::SP model) {
-    auto m = std::dynamic_pointer_cast<ExaStitchModel>(model);
+  bool build(Model::SP model) override {
+    auto m = model ? model->as<ExaStitchModel>() : nullptr;
     if (!m) return false;
+    if (rendererOwned) throw std::runtime_error("this sampler belongs to a renderer");
+    if (!handle) check(exa_sampler_create(&handle));
     const int32_t n[3] = {m->numGridCells.x, m->numGridCells.y, m->numGridCells.z};
     check(exa_model_set_num_grid_cells(m->handle, n));
     float from[6], to[6];
@@ -183,51 +239,144 @@ struct ExaStitchSampler {
     boxArray(m->remapTo, to);
     check(exa_model_set_voxel_space_transform(m->handle, from, to));
     check(exa_sampler_build(handle, m->handle, nullptr));
+    view = handle;
+    fetchAccel();
     return true;
   }
-  void computeMaxOpacities(const std::vector<vec4f> &colorMap, range1f xfRange) {
+  /* the majorants of a renderer's sampler follow the renderer's colour map
+     and range (OWLRenderer::setColorMap/setRange call this themselves) */
+  void computeMaxOpacities(const std::vector<vec4f> &colorMap, range1f xfRange) override {
     if (colorMap.empty()) throw std::runtime_error("empty colour map");
+    if (rendererOwned) throw std::runtime_error("set the colour map and range on the renderer that owns this sampler");
+    if (!handle) throw std::runtime_error("sampler not built");
     check(exa_sampler_compute_max_opacities(handle, &colorMap[0].x, (int)colorMap.size(), xfRange.lower,
                                             xfRange.upper, nullptr));
   }
   // sample() (ExaStitchSampler.h:128-140) over n device-resident points
-  void sample(const float *d_pos, int64_t n, exa_sample *d_out, void *stream = nullptr) const {
-    check(exa_sampler_locate(handle, d_pos, n, d_out, stream));
+  void sample(const float *d_pos, int64_t n, exa_sample *d_out, void *stream = nullptr) const override {
+    check(exa_sampler_locate(live(), d_pos, n, d_out, stream));
   }
-  static std::string className() { return "ExaStitchSampler"; }
+  std::vector<float> maxOpacities() const override {
+    const exa_sampler *s = live();
+    int32_t d[3];
+    check(exa_sampler_get_grid_info(s, d, nullptr));
+    std::vector<float> out((size_t)d[0] * d[1] * d[2]);
+    check(exa_sampler_get_max_opacities(s, out.data()));
+    return out;
+  }
+  std::string className() override { return "ExaStitchSampler"; }
+
+ protected:
+  exa_renderer *createRenderer(Model::SP model, const int32_t numMCs[3], const float from[6],
+                               const float to[6]) override {
+    auto m = model->as<ExaStitchModel>();
+    if (!m) throw std::runtime_error("not an exaStitch model");
+    exa_renderer *r = nullptr;
+    check(exa_renderer_create(m->handle, numMCs, from, to, &r));
+    view = exa_renderer_get_sampler(r);
+    rendererOwned = true;
+    fetchAccel();
+    return r;
+  }
+  void releaseRenderer() override {
+    view = nullptr;
+    majorantAccel.grid.reset();
+  }
+
+ private:
+  const exa_sampler *view = nullptr;  // the built sampler: own or a renderer's
+  bool rendererOwned = false;
+  const exa_sampler *live() const {
+    if (!view) throw std::runtime_error("sampler not built (or its renderer was destroyed)");
+    return view;
+  }
+  void fetchAccel() {
+    auto g = std::make_shared<Grid>();
+    float b[6];
+    check(exa_sampler_get_grid_info(view, &g->dims.x, b));
+    g->bounds = {{b[0], b[1], b[2]}, {b[3], b[4], b[5]}};
+    majorantAccel.grid = g;
+    majorantAccel.kdtree.reset();
+  }
 };
 
 /* sampler/ExaBrickSampler in its default configuration (ABR traversal, ABR
    sampling; ExaBrickSampler.cpp:26-247): build() makes the active brick
    regions' device tables, computeMaxOpacities the per-region majorants
    (regions at 0 are culled), sample() the region blend by point. */
-struct ExaBrickSampler {
+struct ExaBrickSampler : Sampler {
   typedef std::shared_ptr<ExaBrickSampler> SP;
-  exa_brick_sampler *handle = nullptr;
+  exa_brick_sampler *handle = nullptr;  // built by build(); owned
   ExaBrickSampler() = default;
-  ~ExaBrickSampler() { exa_brick_sampler_destroy(handle); }
+  ~ExaBrickSampler() override { exa_brick_sampler_destroy(handle); }
   ExaBrickSampler(const ExaBrickSampler &) = delete;
   ExaBrickSampler &operator=(const ExaBrickSampler &) = delete;
-  bool build(Model::SP model) {
-    auto m = std::dynamic_pointer_cast<ExaBrickModel>(model);
+  bool build(Model::SP model) override {
+    auto m = model ? model->as<ExaBrickModel>() : nullptr;
     if (!m) return false;
+    if (rendererOwned) throw std::runtime_error("this sampler belongs to a renderer");
     exa_brick_sampler_destroy(handle);
     handle = nullptr;
+    view = nullptr;
     check(exa_brick_sampler_create(m->handle, nullptr, &handle));
+    view = handle;
+    fetchAccel(*m);
     return true;
   }
-  void computeMaxOpacities(const std::vector<vec4f> &colorMap, range1f xfRange) {
+  void computeMaxOpacities(const std::vector<vec4f> &colorMap, range1f xfRange) override {
     if (colorMap.empty()) throw std::runtime_error("empty colour map");
+    if (rendererOwned) throw std::runtime_error("set the colour map and range on the renderer that owns this sampler");
     if (!handle) throw std::runtime_error("sampler not built");
     check(exa_brick_sampler_compute_max_opacities(handle, &colorMap[0].x, (int)colorMap.size(), xfRange.lower,
                                                   xfRange.upper, nullptr));
   }
   // sample() by point (ExaBrickSampler.h:221-236) over n device-resident points
-  void sample(const float *d_pos, int64_t n, exa_sample *d_out, void *stream = nullptr) const {
-    if (!handle) throw std::runtime_error("sampler not built");
-    check(exa_brick_sampler_locate(handle, d_pos, n, d_out, stream));
+  void sample(const float *d_pos, int64_t n, exa_sample *d_out, void *stream = nullptr) const override {
+    check(exa_brick_sampler_locate(live(), d_pos, n, d_out, stream));
   }
-  static std::string className() { return "ExaBrickSampler"; }
+  std::vector<float> maxOpacities() const override {
+    const exa_brick_sampler *s = live();
+    std::vector<float> out((size_t)numRegions);
+    check(exa_brick_sampler_get_max_opacities(s, out.data()));
+    return out;
+  }
+  std::string className() override { return "ExaBrickSampler"; }
+
+ protected:
+  exa_renderer *createRenderer(Model::SP model, const int32_t numMCs[3], const float from[6],
+                               const float to[6]) override {
+    auto m = model->as<ExaBrickModel>();
+    if (!m) throw std::runtime_error("not an ExaBricks model");
+    exa_renderer *r = nullptr;
+    check(exa_renderer_create_bricks(m->handle, numMCs, from, to, &r));
+    view = exa_renderer_get_brick_sampler(r);
+    rendererOwned = true;
+    fetchAccel(*m);
+    return r;
+  }
+  void releaseRenderer() override {
+    view = nullptr;
+    majorantAccel.kdtree.reset();
+  }
+
+ private:
+  const exa_brick_sampler *view = nullptr;  // the built sampler: own or a renderer's
+  bool rendererOwned = false;
+  int64_t numRegions = 0;
+  const exa_brick_sampler *live() const {
+    if (!view) throw std::runtime_error("sampler not built (or its renderer was destroyed)");
+    return view;
+  }
+  void fetchAccel(const ExaBrickModel &m) {
+    int64_t c[6];  // bricks, cells, regions, leaf list, kd nodes, kd depth
+    check(exa_brick_model_get_counts(m.handle, c));
+    numRegions = c[2];
+    auto k = std::make_shared<KDTree>();
+    k->numNodes = c[4];
+    k->depth = c[5];
+    majorantAccel.kdtree = k;
+    majorantAccel.grid.reset();
+  }
 };
 
 struct OWLRenderer {
@@ -279,7 +428,10 @@ struct OWLRenderer {
     throw std::runtime_error("block array constructor not implemented yet");
   }
 
-  ~OWLRenderer() { exa_renderer_destroy(r); }
+  ~OWLRenderer() {
+    if (sampler) sampler->releaseRenderer();
+    exa_renderer_destroy(r);
+  }
   OWLRenderer(const OWLRenderer &) = delete;
   OWLRenderer &operator=(const OWLRenderer &) = delete;
 
@@ -380,6 +532,7 @@ struct OWLRenderer {
 
   Type type = Type::PathTracer;
   Model::SP model;
+  Sampler::SP sampler;
   exa_renderer *r = nullptr;
   int accumID = 0;
   vec2i fbSize{1, 1};
@@ -398,10 +551,14 @@ struct OWLRenderer {
     model->setVoxelSpaceTransform(remap_from, remap_to);
     model->setNumGridCells(numMCs);
     const int32_t n[3] = {numMCs.x, numMCs.y, numMCs.z};
-    if (auto m = std::dynamic_pointer_cast<ExaStitchModel>(model))
-      check(exa_renderer_create(m->handle, n, from, to, &r));
-    else if (auto b = std::dynamic_pointer_cast<ExaBrickModel>(model))
-      check(exa_renderer_create_bricks(b->handle, n, from, to, &r));
+    // initGPU (OWLRenderer.cpp:315-337): the sampler by the model's type,
+    // which then makes the renderer whose launch reads its device data
+    if (model->as<ExaBrickModel>())
+      sampler = std::make_shared<ExaBrickSampler>();
+    else if (model->as<ExaStitchModel>())
+      sampler = std::make_shared<ExaStitchSampler>();
+    if (!sampler) throw std::runtime_error("Could not load module");
+    r = sampler->createRenderer(model, n, from, to);
     float mb[6], vr[2];
     check(exa_renderer_get_bounds(r, mb));
     check(exa_renderer_get_value_range(r, vr));

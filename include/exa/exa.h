@@ -249,6 +249,8 @@ int exa_renderer_get_accum_id(const exa_renderer *r);
 /* OWLRenderer::accumID is a public field (OWLRenderer.h:163); the next
    render() seeds its RNG with (pixelID, accumID) */
 int exa_renderer_set_accum_id(exa_renderer *r, int accumID);
+/* the exaStitch sampler the renderer built (its OWLRenderer::sampler,
+   OWLRenderer.h:171); valid while the renderer lives */
 const exa_sampler *exa_renderer_get_sampler(const exa_renderer *r);
 /* selects the sampler's accelerator (building it from the renderer's model
    copy when needed) */

@@ -48,10 +48,11 @@ void checkBrick(const Brick &b) {
     const int64_t hi = (int64_t)b.lower[k] + ((int64_t)b.size[k] << b.level);
     if (hi > INT32_MAX) throw std::runtime_error("brick extends beyond the int32 coordinate range");
   }
-  // cell counts are 32-bit offsets (begin); reject before the product can overflow
+  // the device reads a brick's scalar offset (begin) and its in-brick cell
+  // index as int32: reject before the product can overflow
   const int64_t xy = (int64_t)b.size[0] * b.size[1];
-  if (xy > (int64_t)UINT32_MAX || xy * b.size[2] > (int64_t)UINT32_MAX)
-    throw std::runtime_error("brick holds more than 2^32 cells");
+  if (xy > (int64_t)INT32_MAX || xy * b.size[2] > (int64_t)INT32_MAX)
+    throw std::runtime_error("brick holds more than 2^31-1 cells");
 }
 
 // A brick's domain clipped to a region (ABRs.cpp buildPrims)
@@ -279,6 +280,7 @@ void brickModelFromArrays(const Brick *bricks, const float *scalars, size_t numB
   if (numBricks == 0 || !bricks || !scalars) throw std::runtime_error("no bricks");
   for (size_t i = 0; i < numBricks; ++i) checkBrick(bricks[i]);
   const int64_t numCells = (int64_t)bricks[numBricks - 1].begin + brickCells(bricks[numBricks - 1]);
+  if (numCells > (int64_t)INT32_MAX) throw std::runtime_error("too many brick cells (more than 2^31-1)");
   for (size_t i = 0; i < numBricks; ++i)
     if ((int64_t)bricks[i].begin + brickCells(bricks[i]) > numCells)
       throw std::runtime_error("a brick reaches beyond the last brick's cells");
@@ -301,7 +303,8 @@ void readBrickFile(const std::string &path, std::vector<Brick> &bricks, std::vec
     checkBrick(b);
     b.begin = (uint32_t)cellIDs.size();
     const int64_t n = brickCells(b);
-    if ((int64_t)cellIDs.size() + n > (int64_t)UINT32_MAX) throw std::runtime_error("too many brick cells");
+    if ((int64_t)cellIDs.size() + n > (int64_t)INT32_MAX)
+      throw std::runtime_error("too many brick cells (more than 2^31-1)");
     cellIDs.resize(cellIDs.size() + n);
     in.read((char *)(cellIDs.data() + b.begin), n * sizeof(int32_t));
     if (!in.good()) throw std::runtime_error("truncated brick file " + path);
@@ -326,6 +329,7 @@ void writeBrickFile(const std::string &path, const std::vector<Brick> &bricks, c
 void brickModelFromCellIDs(const std::vector<Brick> &bricks, const std::vector<int32_t> &cellIDs,
                            const float *orderedScalars, int64_t numScalars, BrickModel &m) {
   if (bricks.empty()) throw std::runtime_error("no bricks");
+  if (cellIDs.size() > (size_t)INT32_MAX) throw std::runtime_error("too many brick cells (more than 2^31-1)");
   m.bricks = bricks;
   m.scalars.resize(cellIDs.size());
   // ExaBrickModel.cpp:76-95
@@ -463,6 +467,8 @@ void bricksFromCells(const AmrCell *cells, int64_t n, int blockCells, std::vecto
     }
   });
   for (Out &o : outs) {
+    if ((int64_t)cellIDs.size() + (int64_t)o.ids.size() > (int64_t)INT32_MAX)
+      throw std::runtime_error("too many brick cells (more than 2^31-1)");
     const uint32_t off = (uint32_t)cellIDs.size();
     for (Brick b : o.bricks) {
       b.begin += off;

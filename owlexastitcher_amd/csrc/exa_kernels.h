@@ -3,10 +3,33 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <atomic>
 #include "exa_device.h"
 #include "exa_host.h"
 
 namespace exa {
+
+// Workgroups of `kernel` (at `block` threads) resident on the whole current
+// device: occupancy per CU times the CU count, computed once per device and
+// kept in `cache` (one slot per device ordinal; atomics, so concurrent first
+// calls from several host threads at worst compute it twice).
+constexpr int kMaxCachedDevices = 64;
+template <typename K>
+int residentWorkgroups(K kernel, int block, std::atomic<int> (&cache)[kMaxCachedDevices]) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) dev = -1;
+  if (dev >= 0 && dev < kMaxCachedDevices) {
+    const int v = cache[dev].load(std::memory_order_relaxed);
+    if (v > 0) return v;
+  }
+  int perCU = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, kernel, block, 0) != hipSuccess || perCU < 1) perCU = 1;
+  if (dev < 0 || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+    cus = 256;
+  const int v = perCU * cus;
+  if (dev >= 0 && dev < kMaxCachedDevices) cache[dev].store(v, std::memory_order_relaxed);
+  return v;
+}
 
 // Point-location data on the device (the ExaStitchSampler::LP of
 // sampler/ExaStitchSampler.h:42-49 plus the geometry the OptiX programs

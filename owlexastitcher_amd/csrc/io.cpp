@@ -112,9 +112,13 @@ void readElementMesh(const std::string &path, ElementMesh &m) {
   in.read(h, sizeof(h), "header");
   const int64_t total = in.size();
   fseeko(in.f, 8 + sizeof(h), SEEK_SET);
+  // every count below 2^31 before any product is formed, so the size sum
+  // cannot wrap around to the file length
+  for (int t = 0; t < 5; ++t)
+    if (h[t] >= (1ull << 31)) throw std::runtime_error("'" + path + "': a count in the header exceeds 2^31-1");
   uint64_t need = h[0] * 12 + ((h[5] & 1) ? h[0] * 4 : 0) + ((h[5] & 2) ? h[0] * 4 : 0);
   for (int t = 0; t < 4; ++t) need += h[1 + t] * 4 * kNumVerts[t];
-  if (h[0] >= (1ull << 31) || need + 8 + sizeof(h) != (uint64_t)total)
+  if (need + 8 + sizeof(h) != (uint64_t)total)
     throw std::runtime_error("'" + path + "': sizes in the header do not match the file");
   m.xyz.resize(3 * h[0]);
   in.read(m.xyz.data(), m.xyz.size() * 4, "vertices");

@@ -425,19 +425,8 @@ __global__ __launch_bounds__(kBrickPersistBlock) __attribute__((amdgpu_waves_per
 
 template <bool kCount>
 static int residentBrickBlocks() {
-  static int cached = 0;
-  if (!cached) {
-    int perCU = 0, dev = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, kRenderBricksPersistent<kCount>, kBrickPersistBlock,
-                                                     0) != hipSuccess ||
-        perCU < 1)
-      perCU = 1;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
-      cus = 256;
-    cached = perCU * cus;
-  }
-  return cached;
+  static std::atomic<int> cache[kMaxCachedDevices];
+  return residentWorkgroups(kRenderBricksPersistent<kCount>, kBrickPersistBlock, cache);
 }
 
 void launchBrickLocate(const BrickView &v, const float *pos, int64_t n, Sample *out, hipStream_t s) {

@@ -443,19 +443,8 @@ __global__ void kAdvanceBatch(BatchRequest *reqs, int n) {
 
 template <bool kCount, int kAccel, bool kBatch, bool kNeural>
 static int residentBlocks() {
-  static int cached = 0;
-  if (!cached) {
-    int perCU = 0, dev = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, kRenderPersistent<kCount, kAccel, kBatch, kNeural>,
-                                                     kPersistBlock, 0) != hipSuccess ||
-        perCU < 1)
-      perCU = 1;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
-      cus = 256;
-    cached = perCU * cus;
-  }
-  return cached;
+  static std::atomic<int> cache[kMaxCachedDevices];
+  return residentWorkgroups(kRenderPersistent<kCount, kAccel, kBatch, kNeural>, kPersistBlock, cache);
 }
 
 template <bool kCount, int kAccel, bool kBatch, bool kNeural = false>

@@ -318,14 +318,21 @@ def test_invalid_bricks_rejected():
         b = np.array([(lower, size, level, 0)], ora.BRICK_DTYPE)
         with pytest.raises(ExaError):
             ExaBrickModel.from_arrays(b, s)
-    # a file whose brick claims more than 2^32 cells fails before allocating
+    # files whose brick claims more than 2^31-1 cells fail before allocating:
+    # the device reads offsets and in-brick cell indices as int32
     import tempfile
     with tempfile.TemporaryDirectory() as d:
         path = os.path.join(d, "huge.bricks")
-        with open(path, "wb") as f:
-            f.write(np.array([65536, 65536, 2, 0, 0, 0, 0], np.int32).tobytes() + bytes(64))
-        with pytest.raises(ExaError):
-            ExaBrickModel.load(path, path)
+        for size in [(65536, 65536, 2), (32768, 32768, 2), (2**31 - 1, 2, 1)]:
+            with open(path, "wb") as f:
+                f.write(np.array(list(size) + [0, 0, 0, 0], np.int32).tobytes() + bytes(64))
+            with pytest.raises(ExaError):
+                ExaBrickModel.load(path, path)
+    # arrays whose last brick ends past 2^31-1 scalars are refused before the
+    # scalars are read
+    b = np.array([((0, 0, 0), (2, 2, 2), 0, 2**31 - 4)], ora.BRICK_DTYPE)
+    with pytest.raises(ExaError):
+        ExaBrickModel.from_arrays(b, s)
     # the largest extent that still fits: upper face at 2^31 - 4
     b = np.array([((2**31 - 8, 0, 0), (2, 2, 2), 1, 0)], ora.BRICK_DTYPE)
     assert ExaBrickModel.from_arrays(b, s).counts()["bricks"] == 1

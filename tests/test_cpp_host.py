@@ -47,6 +47,20 @@ int main(int argc, char **argv) {
   auto bm = ExaBrickModel::load(b.data(), &v, 1);
   std::printf("%d %d\n", (int)bs.build(bm), (int)bs.build(m));
   bs.computeMaxOpacities({{0, 0, 0, 1}, {1, 1, 1, 1}}, bm->valueRange);
+  // a caller written against the plugin base (sampler/Sampler.h:27-69)
+  std::vector<Sampler::SP> plugins = {std::make_shared<ExaBrickSampler>(), std::make_shared<ExaStitchSampler>()};
+  for (const Sampler::SP &sp : plugins) {
+    const bool built = sp->build(m);  // false: not a model of its kind
+    std::printf("%s %d %d", sp->className().c_str(), (int)built, (int)(sp->as<ExaStitchSampler>() != nullptr));
+    if (built) {
+      sp->computeMaxOpacities({{0, 0, 0, 0}, {1, 1, 1, 1}}, m->valueRange);
+      std::printf(" %zu %d", sp->maxOpacities().size(), (int)(sp->majorantAccel.grid != nullptr));
+    }
+    std::printf("\n");
+  }
+  // the renderers picked their samplers by the model type (OWLRenderer.cpp:315-325)
+  std::printf("%s %s %zu\n", r.sampler->className().c_str(), rb.sampler->className().c_str(),
+              rb.sampler->maxOpacities().size());
   return 0;
 }
 '''

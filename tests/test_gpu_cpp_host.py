@@ -81,4 +81,6 @@ def test_cpp_classes_render_and_build(files, tmp_path, device):
                     lib_dir, "-lexastitch", f"-Wl,-rpath,{lib_dir}"], check=True, timeout=300)
     r = subprocess.run([str(exe), p["exum"], p["scalars"]], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
-    assert r.stdout.split() == ["1", "16", "3", "1", "1", "0"]
+    assert r.stdout.split() == ["1", "16", "3", "1", "1", "0",
+                                "ExaBrickSampler", "0", "0", "ExaStitchSampler", "1", "1", "64", "1",
+                                "ExaStitchSampler", "ExaBrickSampler", "1"]

@@ -7,10 +7,9 @@ workload at 1 spp: the oracle's majorant grid and per-primitive culling
 against the library's, then every eighth image row (270 rows, 1.04M pixels)
 against the oracle's accumulation and frame buffers, bit for bit.
 
-The scene build alone takes over a minute, so the file runs only when
-EXA_LARGE_PARITY is set in the environment.
+The scene build alone takes over a minute (about 110 s for the whole file on
+one MI355X), so the test carries its own time limit.
 """
-import os
 import time
 
 import numpy as np
@@ -19,9 +18,7 @@ import pytest
 from oracle import oracle as ora
 from owlexastitcher_amd import AmrScene, DualMesh, ExaStitchModel, OWLRenderer, camera_look_at, tf_rasterize
 
-pytestmark = [pytest.mark.gpu,
-              pytest.mark.skipif(not os.environ.get("EXA_LARGE_PARITY"),
-                                 reason="full-size large scene (over a minute to build); set EXA_LARGE_PARITY=1")]
+pytestmark = [pytest.mark.gpu, pytest.mark.timeout(900)]
 
 TF_SHOCK = np.array([
     [0.00, 0.1, 0.1, 0.3, 0.00],

@@ -214,6 +214,12 @@ def test_model_loader_errors(tmp_path):
         f.write(b"NOTAMESH" + b"\0" * 48)
     with pytest.raises(ExaError, match="not an element-mesh file"):
         ExaStitchModel.load(em, None, None)
+    # an element count whose size wraps the 64-bit byte sum to the file length
+    # (2^60 tets x 16 B = 2^64) is refused, not read as zero elements
+    with open(em, "wb") as f:
+        f.write(b"EXAELEM1" + struct.pack("<6Q", 1, 2 ** 60, 0, 0, 0, 0) + bytes(12))
+    with pytest.raises(ExaError, match="exceeds 2\\^31-1"):
+        ExaStitchModel.load(em, None, None)
     write_grids(gr, [((0, 0, 0), 0, (1, 1, 1), [0] * 8)])
     with open(gr, "ab") as f:
         f.write(np.array([0, 0, 0, 0, 1, 1, 1, 0, 0], np.int32).tobytes())  # second record cut short
