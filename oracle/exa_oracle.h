@@ -134,6 +134,12 @@ int ora_dda_trace_f32(const float org[3], const float dir[3], float tmin, float 
 void ora_tf_lookup_f32(const float *rgba, int32_t N, const float domain[2],
                        float v, float out[4]);
 
+/* ---- phase function (deviceCode.cu:144-149, 957-985) -------------------- */
+void ora_orthonormal_basis_f32(const float w[3], float u[3], float v[3]);
+/* n directions hg_sample(wo) (g = 0) drawn from the LCG stream seeded with
+   (v0, v1), 3 floats each */
+void ora_hg_sample_f32(const float wo[3], uint32_t v0, uint32_t v1, int64_t n, float *out);
+
 /* ---- RNG and math ------------------------------------------------------- */
 uint32_t ora_lcg_init(uint32_t v0, uint32_t v1);
 float    ora_lcg_next(uint32_t *state);

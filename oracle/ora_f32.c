@@ -452,6 +452,22 @@ void ora_tf_lookup_f32(const float *rgba, int32_t N, const float domain[2], floa
   out[0] = r.x; out[1] = r.y; out[2] = r.z; out[3] = r.w;
 }
 
+void ora_orthonormal_basis_f32(const float w[3], float u[3], float v[3]) {
+  V3 uu, vv;
+  orthonormal_basis(&uu, &vv, ld3(w));
+  u[0] = uu.x; u[1] = uu.y; u[2] = uu.z;
+  v[0] = vv.x; v[1] = vv.y; v[2] = vv.z;
+}
+
+void ora_hg_sample_f32(const float wo[3], uint32_t v0, uint32_t v1, int64_t n, float *out) {
+  Rng rng;
+  rng.state = ora_lcg_init(v0, v1);
+  for (int64_t i = 0; i < n; ++i) {
+    const V3 d = hg_sample(ld3(wo), &rng);
+    out[3 * i] = d.x; out[3 * i + 1] = d.y; out[3 * i + 2] = d.z;
+  }
+}
+
 /* ===================================================================== */
 /* model/ExaStitchModel.cpp:78-99,224-245                                  */
 /* ===================================================================== */

@@ -79,6 +79,8 @@ _SIGS = {
     "ora_element_max_opacity": (None, [P, P, C.c_int32, P, P]),
     "ora_dda_trace_f32": (C.c_int, [P, P, C.c_float, C.c_float, P, P, P, P, P, C.c_int]),
     "ora_tf_lookup_f32": (None, [P, C.c_int32, P, C.c_float, P]),
+    "ora_orthonormal_basis_f32": (None, [P, P, P]),
+    "ora_hg_sample_f32": (None, [P, C.c_uint32, C.c_uint32, C.c_int64, P]),
     "ora_lcg_init": (C.c_uint32, [C.c_uint32, C.c_uint32]),
     "ora_lcg_next": (C.c_float, [P]),
     "ora_logf": (C.c_float, [C.c_float]),
@@ -246,6 +248,21 @@ def tf_lookup(rgba, domain, v):
     dom = np.ascontiguousarray(domain, np.float32)
     out = np.zeros(4, np.float32)
     lib().ora_tf_lookup_f32(ptr(rgba), len(rgba), ptr(dom), float(v), ptr(out))
+    return out
+
+
+def orthonormal_basis(w):
+    w = np.ascontiguousarray(w, np.float32)
+    u = np.zeros(3, np.float32)
+    v = np.zeros(3, np.float32)
+    lib().ora_orthonormal_basis_f32(ptr(w), ptr(u), ptr(v))
+    return u, v
+
+
+def hg_sample(wo, v0, v1, n):
+    wo = np.ascontiguousarray(wo, np.float32)
+    out = np.zeros((n, 3), np.float32)
+    lib().ora_hg_sample_f32(ptr(wo), v0, v1, n, ptr(out))
     return out
 
 

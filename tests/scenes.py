@@ -48,5 +48,40 @@ def noise_small(seed=5):
     return Scene(AmrScene.noise(16, 2, 0.55, 0.7, 6, 2.2, seed))
 
 
+def octree(seed=1, split=0.4, affine=None):
+    """Unbalanced AMR: 4^3 blocks of width-8 cells over [0,32]^3, each split
+    recursively (probability `split` per cell, down to width 1) with no 2:1
+    balance, so neighbouring leaves differ by up to three levels.  Its dual
+    mesh has tetrahedra: dual cells whose top collapses to one vertex and
+    whose base loses an edge (makeDual.cpp:512-528).  Scalars are uniform
+    random, or the affine field a.xyz + b at the cell centre when
+    `affine` = (a, b)."""
+    rng = np.random.default_rng(seed)
+    cells = []
+
+    def rec(x, y, z, level):
+        if level > 0 and rng.random() < split:
+            h = 1 << (level - 1)
+            for dz in (0, h):
+                for dy in (0, h):
+                    for dx in (0, h):
+                        rec(x + dx, y + dy, z + dz, level - 1)
+        else:
+            cells.append((x, y, z, level))
+
+    for i in range(4):
+        for j in range(4):
+            for k in range(4):
+                rec(8 * i, 8 * j, 8 * k, 3)
+    cells = np.array(cells, np.int32)
+    if affine is None:
+        scalars = rng.random(len(cells)).astype(np.float32)
+    else:
+        a, b = affine
+        centre = cells[:, :3].astype(np.float64) + 0.5 * (1 << cells[:, 3])[:, None].astype(np.float64)
+        scalars = (centre @ np.asarray(a, np.float64) + b).astype(np.float32)
+    return Scene(AmrScene.from_cells(cells, scalars))
+
+
 def tf128(points=TF5):
     return tf_rasterize(points, 128)
