@@ -58,7 +58,10 @@ int main(int argc, char **argv) {
     }
     std::printf("\n");
   }
-  // the renderers picked their samplers by the model type (OWLRenderer.cpp:315-325)
+  // the renderers picked their samplers by the model type (OWLRenderer.cpp:315-325);
+  // a renderer's sampler maps its majorants from the renderer's colour map
+  rb.setColorMap({{0, 0, 0, 1}, {1, 1, 1, 1}});
+  rb.setRange(rb.valueRange);
   std::printf("%s %s %zu\n", r.sampler->className().c_str(), rb.sampler->className().c_str(),
               rb.sampler->maxOpacities().size());
   return 0;

@@ -1,12 +1,13 @@
 """Batched requests in one persistent launch: every request's image equals
-the single-view renderer's image for the same settings (which is itself
-checked bit for bit against the oracle in test_gpu_render.py), under any
-scheduling knobs, eagerly and through a replayed HIP graph."""
+the oracle's image for its settings and the single-view renderer's, bit for
+bit, under any scheduling knobs, eagerly and through a replayed HIP graph."""
 import numpy as np
 import pytest
 
+from oracle import oracle as ora
 from owlexastitcher_amd import OWLRenderer, RenderBatch, camera_look_at
 from tests import scenes
+from tests.test_gpu_render import oracle_model
 
 pytestmark = pytest.mark.gpu
 
@@ -88,6 +89,44 @@ def single_images(r, tfs, frames):
         torch.cuda.synchronize()
         out.append((r.accum(), fb.cpu().numpy().view(np.uint32)))
     return out
+
+
+def oracle_images(sc, r, tfs, frames):
+    """each request rendered by the oracle with the settings the single-view
+    renderer takes for it (TF culling and majorants of its TF)"""
+    out = []
+    ranges = None
+    for q in REQUESTS:
+        configure_single(r, q, tfs)
+        r.accumID = 0
+        st = r.state()
+        rgba = tfs[q[4]]
+        xr = np.array(st.tfDomain, np.float32)
+        om = oracle_model(sc, rgba, xr)
+        if ranges is None:
+            ranges = om.grid_build(np.array(st.gridDims, np.int32), np.array(st.gridBounds, np.float32))
+        maj = ora.map_max_opacity(ranges, rgba, xr)
+        acc = None
+        for f in range(frames):
+            p = ora.render_params_from_state(st, maj, rgba)
+            p.accumID = f
+            acc, fb = ora.render_f32(p, om, acc)
+        out.append((acc, fb))
+    return out
+
+
+def test_batch_matches_oracle(cfg1, batch_setup):
+    import torch
+    print("case: batch of 6 requests against the oracle, 2 frames", flush=True)
+    r, b, tfs = batch_setup
+    b.resetAccum()
+    for _ in range(2):
+        b.render()
+    torch.cuda.synchronize()
+    for i, (acc_o, fb_o) in enumerate(oracle_images(cfg1, r, tfs, 2)):
+        acc_b = b.accum(i)
+        assert np.array_equal(acc_b, acc_o), f"request {i}: {np.count_nonzero(np.any(acc_b != acc_o, -1))} px differ"
+        assert np.array_equal(b.frame(i), fb_o)
 
 
 def test_batch_matches_single_view(batch_setup):

@@ -113,11 +113,16 @@ def test_batch_workload_matches_single_view(workload, device):
     """The batched-requests workload: 64 requests of sizes drawn from
     {256, 512, 768, 1024}^2 with seeded orbit cameras and 4 random 5-8 point
     TFs, 16 spp, one persistent launch; four of the requests (the first of
-    each size) against the single-view renderer with the same settings, whose
-    frames the tests above check against the oracle."""
+    each size) against the single-view renderer with the same settings, and
+    every fourth row of each against the oracle, bit for bit."""
+    import ctypes as C
+    from concurrent.futures import ThreadPoolExecutor
     import torch
     from owlexastitcher_amd import RenderBatch, tf_rasterize
-    r, _, _, _ = workload
+    r, om, _, rgba0 = workload
+    st0 = r.state()
+    xr0 = np.array(st0.tfDomain, np.float32)
+    r_culling = (ora.map_max_opacity(om.gridlet_value_ranges(), rgba0, xr0), om.element_max_opacity(rgba0, xr0))
     print("case: batch workload, 64 requests, 16 spp, 1 frame", flush=True)
     rng = np.random.default_rng(1)
     vr = r.valueRange
@@ -154,21 +159,47 @@ def test_batch_workload_matches_single_view(workload, device):
     r.setScheme(RENDER_PERSISTENT)
     r.setSpp(16)
     r.setType(0)
-    for i in picks:
-        size, tf, cam = reqs[i]
-        r.resize(size, size)
-        r.setCamera(*cam)
-        r.setColorMap(tfs[tf][0])
-        r.setRange(vr[0], vr[1])
-        r.setOpacityScale(tfs[tf][1])
-        r.resetAccum()
-        fb = torch.zeros((size, size), dtype=torch.int32, device=device)
-        r.render(fb)
-        torch.cuda.synchronize()
-        acc_s, acc_b = r.accum(), bt.accum(i)
-        assert np.array_equal(acc_b, acc_s), f"request {i}: {np.count_nonzero(np.any(acc_b != acc_s, -1))} px differ"
-        assert np.array_equal(bt.frame(i), fb.cpu().numpy().view(np.uint32))
-        assert not np.all(acc_b == acc_b[:1, :1])
+    om.bins()
+    ranges = None
+    try:
+        for i in picks:
+            size, tf, cam = reqs[i]
+            r.resize(size, size)
+            r.setCamera(*cam)
+            r.setColorMap(tfs[tf][0])
+            r.setRange(vr[0], vr[1])
+            r.setOpacityScale(tfs[tf][1])
+            r.resetAccum()
+            st = r.state()
+            fb = torch.zeros((size, size), dtype=torch.int32, device=device)
+            r.render(fb)
+            torch.cuda.synchronize()
+            acc_s, acc_b = r.accum(), bt.accum(i)
+            assert np.array_equal(acc_b, acc_s), f"request {i}: {np.count_nonzero(np.any(acc_b != acc_s, -1))} px"
+            assert np.array_equal(bt.frame(i), fb.cpu().numpy().view(np.uint32))
+            assert not np.all(acc_b == acc_b[:1, :1])
+            # the batched frame against the oracle itself, every fourth row,
+            # under this request's TF culling and majorants
+            t_rgba = tfs[tf][0]
+            xr = np.array(st.tfDomain, np.float32)
+            om.set_culling(ora.map_max_opacity(om.gridlet_value_ranges(), t_rgba, xr),
+                           om.element_max_opacity(t_rgba, xr))
+            if ranges is None:
+                ranges = om.grid_build(np.array(st.gridDims, np.int32), np.array(st.gridBounds, np.float32))
+            maj = ora.map_max_opacity(ranges, t_rgba, xr)
+            p = ora.render_params_from_state(st, maj, t_rgba)
+            acc_o = np.zeros((size, size, 4), np.float32)
+            fb_o = np.zeros((size, size), np.uint32)
+            rows = list(range(i % 4, size, 4))
+            with ThreadPoolExecutor(16) as ex:
+                list(ex.map(lambda y: ora.lib().ora_render_f32(C.byref(p), om.ref, om.bins(), ora.ptr(acc_o),
+                                                                ora.ptr(fb_o), y, y + 1), rows))
+            npx = np.count_nonzero(np.any(acc_b[rows] != acc_o[rows], axis=-1))
+            print(f"  request {i} ({size}^2, TF {tf}): {npx} of {len(rows) * size} oracle pixels differ", flush=True)
+            assert npx == 0
+            assert np.array_equal(bt.frame(i)[rows], fb_o[rows])
+    finally:
+        om.set_culling(*r_culling)
 
 
 def test_bricks_workload_full_size(amr, workload, device):
