@@ -93,7 +93,17 @@ def algorithmic_bytes(c):
     return sum(BYTES[k] * c[k] for k in BYTES)
 
 
-def cpu_baseline(r, rgba, dm, args, target_s):
+def host_threads():
+    """host threads for the CPU baseline: the CPUs this process may run on,
+    at most 16 (the host share of one GPU)"""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_baseline(r, rgba, dm, args, target_s, dev):
     """The oracle (single-precision restatement of the reference's sampling
     and integration code) on this host's cores: whole frames of the same
     workload, consecutive accumIDs, until about target_s seconds of work."""
@@ -108,7 +118,7 @@ def cpu_baseline(r, rgba, dm, args, target_s):
     bins = om.bins(64)
     log(f"[bench] cpu baseline: oracle bins built ({time.time() - t:.1f}s)")
     maj = r.sampler.max_opacities()
-    threads = os.cpu_count() or 1
+    threads = host_threads()
     accum = np.zeros((st.H, st.W, 4), np.float32)
     fb = np.zeros((st.H, st.W), np.uint32)
     frames, elapsed = 0, 0.0
@@ -122,10 +132,99 @@ def cpu_baseline(r, rgba, dm, args, target_s):
             elapsed += time.time() - t
             frames += 1
     samples = frames * st.H * st.W * st.spp
-    return {"value": samples / elapsed, "unit": "samples/s", "cores": threads, "kind": "port",
-            "sample": f"{frames} whole {st.W}x{st.H} frames ({st.spp} spp, accumIDs 0..{frames - 1}) of the same "
-                      f"workload, oracle f32 restatement, {threads} host threads (one image row per task), "
-                      f"{elapsed:.1f}s"}
+    out = {"value": samples / elapsed, "unit": "samples/s", "cores": threads, "kind": "port",
+           "sample": f"{frames} whole {st.W}x{st.H} frames ({st.spp} spp, accumIDs 0..{frames - 1}) of the same "
+                     f"workload, oracle f32 restatement, {threads} host threads (one image row per task), "
+                     f"{elapsed:.1f}s", "cpu_model": cpu_model_name()}
+    try:
+        out["point_location"] = cpu_locate_baseline(r, om, maj, rgba, args, dev, threads)
+    except Exception as e:  # reported, not required
+        log(f"[bench] cpu locate baseline failed: {e!r}")
+    return out
+
+
+def cpu_model_name():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _time_locate(om, pos, threads):
+    """wall seconds of the oracle's point location (its binned search over
+    gridlets and elements, values interpolated) over pos on `threads` host
+    threads, and the samples it returned"""
+    from concurrent.futures import ThreadPoolExecutor
+    chunk = max(1, -(-len(pos) // (threads * 8)))
+    parts = [pos[i:i + chunk] for i in range(0, len(pos), chunk)]
+    om.bins()
+    t = time.perf_counter()
+    if threads == 1:
+        out = [om.locate_binned(p) for p in parts]
+    else:
+        with ThreadPoolExecutor(threads) as ex:
+            out = list(ex.map(om.locate_binned, parts))
+    return time.perf_counter() - t, np.concatenate(out)
+
+
+def cpu_locate_baseline(r, om, maj, rgba, args, dev, threads):
+    """Point location on the host (the CPU sampling path: the oracle's
+    restatement of Gridlet.h / Plane.h / UElems.h behind uniform bins) on
+    the query sets the survey plans: (i) 1M uniform points in [0,128]^3 on
+    the 2-level Gaussian scene, (ii) the queries of the first 1080p frame of
+    the timed workload (recorded by the oracle, whose frame equals the GPU's
+    bit for bit), replayed through point location alone; all threads and one
+    thread (one thread on a bounded prefix).  The GPU locates the same
+    replayed positions, and its samples must equal the host's."""
+    import torch
+    from oracle import oracle as ora
+    from owlexastitcher_amd import AmrScene, DualMesh
+    res = {"cpu_model": cpu_model_name(), "threads": threads, "host_cpus": os.cpu_count()}
+    # (i) config-1 scene, 1M uniform points
+    amr1 = AmrScene.gaussians(64, 1, 32, 96, 8, 1)
+    idx, verts, gl, gs = DualMesh(amr1).arrays()
+    om1 = ora.Model(verts, idx, gl, gs)
+    pos1 = np.random.default_rng(7).uniform(0.0, 128.0, (1 << 20, 3)).astype(np.float32)
+    t_all, _ = _time_locate(om1, pos1, threads)
+    t_one, _ = _time_locate(om1, pos1, 1)
+    res["uniform_1M_gaussian_scene"] = {
+        "queries": len(pos1), "queries_per_s_all_threads": len(pos1) / t_all,
+        "queries_per_s_1_thread": len(pos1) / t_one, "ns_per_query_1_thread": t_one / len(pos1) * 1e9}
+    log(f"[bench] cpu locate, 1M uniform points (2-level Gaussian scene): "
+        f"{len(pos1) / t_all / 1e6:.1f}M/s on {threads} threads, {len(pos1) / t_one / 1e6:.2f}M/s on 1")
+    # (ii) the first frame's queries
+    st = r.state()
+    st.accumID = 0
+    p = ora.render_params_from_state(st, maj, rgba)
+    t = time.perf_counter()
+    q = ora.render_queries_f32(p, om, 1 << 26, threads=threads)
+    log(f"[bench] cpu locate: first frame's {len(q)} queries recorded ({time.perf_counter() - t:.1f}s)")
+    t_all, ref = _time_locate(om, q, threads)
+    n1 = min(len(q), 1 << 22)
+    t_one, _ = _time_locate(om, q[:n1], 1)
+    d_pos = torch.from_numpy(q).to(dev)
+    d_out = torch.empty((len(q), 3), dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream()
+    r.sampler.locate(d_pos, d_out, stream)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(5):
+        r.sampler.locate(d_pos, d_out, stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    gpu_s = e0.elapsed_time(e1) / 5 * 1e-3
+    same = bool(np.array_equal(d_out.cpu().numpy(), ref))
+    res["first_frame_replay"] = {
+        "queries": len(q), "queries_per_s_all_threads": len(q) / t_all, "queries_per_s_1_thread": n1 / t_one,
+        "ns_per_query_1_thread": t_one / n1 * 1e9, "one_thread_sample": f"first {n1} queries",
+        "gpu_queries_per_s_same_positions": len(q) / gpu_s, "gpu_samples_equal_cpu": same}
+    log(f"[bench] cpu locate, first-frame replay: {len(q) / t_all / 1e6:.1f}M/s on {threads} threads, "
+        f"{n1 / t_one / 1e6:.2f}M/s on 1; GPU {len(q) / gpu_s / 1e9:.2f}G/s on the same positions; "
+        f"samples equal: {same}")
+    return res
 
 
 def copy_calibration(nbytes, dev):
@@ -699,7 +798,7 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         try:
-            cpu = cpu_baseline(r, rgba, dm, args, args.cpu_seconds)
+            cpu = cpu_baseline(r, rgba, dm, args, args.cpu_seconds, dev)
             log(f"[bench] cpu baseline: {cpu}")
         except Exception as e:  # the baseline is reported, not required
             log(f"[bench] cpu baseline failed: {e!r}")

@@ -195,6 +195,11 @@ typedef struct {
    fb is uint32[W*H] (may be NULL). */
 void ora_render_f32(const ora_render_params *p, const ora_model *m, const ora_bins *b,
                     float *accum, uint32_t *fb, int32_t y0, int32_t y1);
+/* ora_render_f32 that also records the position of every point-location
+   query (tentative collision) into out (3 floats each, at most cap; *count,
+   shared by threads, counts all of them) */
+void ora_render_queries_f32(const ora_render_params *p, const ora_model *m, const ora_bins *b, float *accum,
+                            int32_t y0, int32_t y1, float *out, int64_t cap, int64_t *count);
 /* FP64 build: same stream of random numbers, double arithmetic; accum is
    double4[W*H]. */
 void ora_render_f64(const ora_render_params *p, const ora_model *m, const ora_bins *b,

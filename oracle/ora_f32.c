@@ -514,6 +514,14 @@ static uint32_t make_8bit(float f) {
   return (uint32_t)(i < 0 ? 0 : (i > 255 ? 255 : i));
 }
 
+void ora_render_queries_f32(const ora_render_params *p, const ora_model *m, const ora_bins *b, float *accum,
+                            int32_t y0, int32_t y1, float *out, int64_t cap, int64_t *count) {
+  QueryRecorder rec = {out, cap, count};
+  g_queries = &rec;
+  ora_render_f32(p, m, b, accum, NULL, y0, y1);
+  g_queries = NULL;
+}
+
 void ora_render_f32(const ora_render_params *p, const ora_model *m, const ora_bins *b,
                     float *accum, uint32_t *fb, int32_t y0, int32_t y1) {
   for (int py = y0; py < y1; ++py)

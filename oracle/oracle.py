@@ -87,6 +87,7 @@ _SIGS = {
     "ora_sincosf": (None, [C.c_float, P, P]),
     "ora_render_f32": (None, [P, P, P, P, P, C.c_int32, C.c_int32]),
     "ora_render_f64": (None, [P, P, P, P, C.c_int32, C.c_int32]),
+    "ora_render_queries_f32": (None, [P, P, P, P, C.c_int32, C.c_int32, P, C.c_int64, P]),
     "ora_dual_build": (P, [P, P, C.c_int64]),
     "ora_dual_count": (C.c_int64, [P, C.c_int]),
     "ora_dual_get": (None, [P, P, P, P, P]),
@@ -399,6 +400,28 @@ def render_f32(params, model, accum=None, threads=8):
     with ThreadPoolExecutor(threads) as ex:
         list(ex.map(work, range(0, H, rows)))
     return accum, fb
+
+
+def render_queries_f32(params, model, cap, threads=8):
+    """One frame as render_f32, returning the positions of its point-location
+    queries (n x 3 float32; order across threads unspecified)."""
+    from concurrent.futures import ThreadPoolExecutor
+    W, H = params.W, params.H
+    accum = np.zeros((H, W, 4), np.float32)
+    out = np.zeros((cap, 3), np.float32)
+    count = np.zeros(1, np.int64)
+    bins = model.bins()
+
+    def work(y):
+        lib().ora_render_queries_f32(C.byref(params), model.ref, bins, ptr(accum), y, y + 1, ptr(out), cap,
+                                     ptr(count))
+
+    with ThreadPoolExecutor(threads) as ex:
+        list(ex.map(work, range(H)))
+    n = int(count[0])
+    if n > cap:
+        raise ValueError(f"frame made {n} queries, more than cap={cap}")
+    return out[:n]
 
 
 def render_f64(params, model, accum=None, threads=8):
