@@ -168,6 +168,17 @@ int exa_sampler_locate(const exa_sampler *s, const float *d_pos, int64_t n, exa_
    pyramid, wedge, hex, miss (synchronises the stream). */
 int exa_sampler_locate_counted(const exa_sampler *s, const float *d_pos, int64_t n, exa_sample *d_out,
                                uint64_t counters[6], void *stream);
+/* exa_sampler_locate without any allocation, for callers that cannot
+   allocate (code under HIP graph capture, tight serving loops, harnesses that
+   forbid hipMalloc); no counterpart in the reference.  The queue of the
+   element stage is a library-owned device array of EXA_LOCATE_NOALLOC_MAX + 1
+   uint32 (64 MiB per GPU, held until the process ends); n beyond the maximum
+   fails with EXA_ERR_INVALID.  Calls that overlap on one GPU (other streams,
+   other host threads) share the array and race; results equal
+   exa_sampler_locate's. */
+#define EXA_LOCATE_NOALLOC_MAX (1 << 24)
+int exa_sampler_locate_noalloc(const exa_sampler *s, const float *d_pos, int64_t n, exa_sample *d_out,
+                               void *stream);
 /* device testSample() (ExaStitchSampler.h:109-126) with per-point gridlet ids. */
 int exa_sampler_test_sample(const exa_sampler *s, const float *d_pos, const int32_t *d_primIDs, int64_t n,
                             exa_sample *d_out, void *stream);

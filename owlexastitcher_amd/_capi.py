@@ -116,6 +116,7 @@ _SIGS = {
     "exa_sampler_compute_max_opacities": (C.c_int, [P, P, C.c_int, F, F, P]),
     "exa_sampler_locate": (C.c_int, [P, P, I64, P, P]),
     "exa_sampler_locate_counted": (C.c_int, [P, P, I64, P, P, P]),
+    "exa_sampler_locate_noalloc": (C.c_int, [P, P, I64, P, P]),
     "exa_sampler_test_sample": (C.c_int, [P, P, P, I64, P, P]),
     "exa_sampler_get_grid_info": (C.c_int, [P, P, P]),
     "exa_sampler_get_value_ranges": (C.c_int, [P, P]),

@@ -7,6 +7,7 @@
 #include <exception>
 #include <fstream>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -943,6 +944,44 @@ int exa_sampler_locate_counted(const exa_sampler *s, const float *d_pos, int64_t
     HIP_CHECK(hipMemcpyAsync(h, s->counters.ptr, sizeof(h), hipMemcpyDeviceToHost, st));
     HIP_CHECK(hipStreamSynchronize(st));
     for (int i = 0; i < 6; ++i) counters[i] = h[i];
+  });
+}
+
+// Point location that allocates nothing: the element-stage queue lives in a
+// file-scope device array instead of stream-ordered scratch.  Each device
+// holds its own copy of the array; host code reaches it only through the
+// address hipGetSymbolAddress gives for that device (the array's name in host
+// code is a stand-in, not a device address).
+static __device__ __align__(256) uint32_t scratch[EXA_LOCATE_NOALLOC_MAX + 1];
+
+static uint32_t *noAllocQueue() {
+  static std::mutex mu;
+  static uint32_t *ptrs[kMaxCachedDevices] = {};
+  int dev = -1;
+  HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxCachedDevices) throw Status(EXA_ERR_STATE, "device ordinal beyond 63");
+  std::lock_guard<std::mutex> lock(mu);
+  if (!ptrs[dev]) {
+    void *p = nullptr;
+    HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) throw Status(EXA_ERR_HIP, "hipGetSymbolAddress returned no address for the locate queue");
+    ptrs[dev] = (uint32_t *)p;
+  }
+  return ptrs[dev];
+}
+
+int exa_sampler_locate_noalloc(const exa_sampler *s, const float *d_pos, int64_t n, exa_sample *d_out,
+                               void *stream) {
+  return guarded([&] {
+    require(s && (n == 0 || (d_pos && d_out)) && n >= 0, "invalid arguments");
+    if (n > (int64_t)EXA_LOCATE_NOALLOC_MAX)
+      throw Status(EXA_ERR_INVALID, "exa_sampler_locate_noalloc: more than EXA_LOCATE_NOALLOC_MAX (2^24) points; "
+                                    "split the batch");
+    if (!s->built) throw Status(EXA_ERR_STATE, "sampler not built");
+    if (n == 0) return;
+    uint32_t *queue = noAllocQueue();  // queue[0] is cleared by a kernel of the launch sequence
+    launchLocate(s->view(), s->accel, d_pos, n, (Sample *)d_out, nullptr, (hipStream_t)stream, queue);
+    HIP_CHECK(hipGetLastError());
   });
 }
 int exa_sampler_test_sample(const exa_sampler *s, const float *d_pos, const int32_t *d_prim, int64_t n,
