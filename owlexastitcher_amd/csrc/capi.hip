@@ -486,6 +486,42 @@ struct exa_neural {
   }
 };
 
+// Buffers of the element-stage queue, sized on use: `depth` entries, or
+// (depth 0) one per pixel of the frame, at most 2^21; a path that finds the
+// queue full runs the renderer's element pass instead.
+constexpr uint32_t kQueueCapMax = 1u << 21;
+constexpr uint32_t kQueueBuckets = 1u << 16;
+struct QueueStore {
+  DeviceBuffer<float4> rec[2];
+  DeviceBuffer<Sample> samples[2];
+  DeviceBuffer<uint32_t> keys[2], order, hist;
+  DeviceBuffer<uint32_t> count;
+  ElementQueueBuffers view(int64_t pixels, int32_t depth) {
+    const int64_t want = depth > 0 ? (int64_t)depth : std::min<int64_t>(pixels, kQueueCapMax);
+    const uint32_t cap = (uint32_t)std::max<int64_t>(1, want);
+    for (int k = 0; k < 2; ++k) {
+      rec[k].alloc((size_t)cap * kElementQueueVec);
+      samples[k].alloc(cap);
+      keys[k].alloc(cap);
+    }
+    order.alloc(cap);
+    hist.alloc(kQueueBuckets);
+    count.alloc(3);
+    ElementQueueBuffers b;
+    for (int k = 0; k < 2; ++k) {
+      b.rec[k] = rec[k].ptr;
+      b.samples[k] = samples[k].ptr;
+      b.keys[k] = keys[k].ptr;
+    }
+    b.order = order.ptr;
+    b.hist = hist.ptr;
+    b.maxBuckets = kQueueBuckets;
+    b.count = count.ptr;
+    b.cap = cap;
+    return b;
+  }
+};
+
 struct exa_renderer {
   HostModel model;
   exa_sampler sampler;
@@ -503,6 +539,7 @@ struct exa_renderer {
   exa_neural *neural = nullptr;  // neural field for validated macrocells (not owned)
   PersistTuning tuning;
   DeviceBuffer<uint32_t> work;
+  QueueStore queue;
   // set for a renderer over an ExaBricks model (then `sampler` stays unbuilt)
   std::unique_ptr<exa_brick_sampler> bricks;
 
@@ -552,6 +589,8 @@ struct exa_batch {
   uint64_t majGeneration = 0;
   DeviceBuffer<uint32_t> work;
   uint32_t numWork = 0;
+  int64_t numPixels = 0;
+  QueueStore queue;
   bool dirty = true;
   PersistTuning tuning;
   hipGraph_t graph = nullptr;
@@ -569,7 +608,9 @@ struct exa_batch {
     if (!dirty) return;
     if (host.empty()) throw Status(EXA_ERR_STATE, "batch has no requests");
     uint32_t w = 0;
+    numPixels = 0;
     for (BatchRequest &b : host) {
+      numPixels += (int64_t)b.P.W * b.P.H;
       b.workBegin = w;
       const uint64_t tiles = (uint64_t)b.tilesX * (uint64_t)((b.P.H + 7) / 8);
       if ((uint64_t)w + tiles * 64 >= (1ull << 32)) throw Status(EXA_ERR_INVALID, "batch too large");
@@ -608,8 +649,12 @@ struct exa_batch {
     const exa_neural *n = activeNeural();
     NeuralView nv{};
     if (n) nv = n->view();
+    ElementQueueBuffers qb{};
+    const bool queued = tuning.queueRounds > 0 && !n && r->sampler.accel == kAccelTables;
+    if (queued) qb = queue.view(numPixels, tuning.queueDepth);
     launchRenderBatch(r->sampler.view(), r->sampler.accel, r->sampler.gridView(), dev.ptr, (int)host.size(),
-                      numWork, work.ptr, tuning, counters, s, n ? &nv : nullptr, n ? n->flags.ptr : nullptr);
+                      numWork, work.ptr, tuning, counters, s, n ? &nv : nullptr, n ? n->flags.ptr : nullptr,
+                      queued ? &qb : nullptr);
   }
   void advanced() {
     for (BatchRequest &b : host) b.P.accumID += 1;
@@ -1191,8 +1236,12 @@ static void doRender(exa_renderer *r, uint32_t *fb, unsigned long long *counters
       throw Status(EXA_ERR_STATE, "the neural field runs with the table accelerator");
     NeuralView nv{};
     if (useNet) nv = r->neural->view();
+    ElementQueueBuffers qb{};
+    const bool queued = r->tuning.queueRounds > 0 && !useNet && r->sampler.accel == kAccelTables;
+    if (queued) qb = r->queue.view((int64_t)P.W * P.H, r->tuning.queueDepth);
     launchRenderPersistent(P, r->sampler.view(), r->sampler.accel, r->sampler.gridView(), r->work.ptr, r->tuning,
-                           counters, s, useNet ? &nv : nullptr, useNet ? r->neural->flags.ptr : nullptr);
+                           counters, s, useNet ? &nv : nullptr, useNet ? r->neural->flags.ptr : nullptr,
+                           queued ? &qb : nullptr);
   } else {
     if (r->neural && r->neural->validated && P.integrator != 2)
       throw Status(EXA_ERR_STATE, "the neural field needs the persistent scheme");
@@ -1409,6 +1458,30 @@ int exa_renderer_set_persistent_tuning(exa_renderer *r, int ctrlSteps, int elemB
     r->tuning.rareBatch = rareBatch;
   });
 }
+static void checkLaunchTuning(int queueRounds, int queueDepth, int blockSize) {
+  require(queueRounds >= 0 && queueRounds <= kMaxQueueRounds, "queueRounds must be in [0, 32]");
+  require(queueDepth >= 0 && queueDepth <= (1 << 24), "queueDepth must be in [0, 2^24]");
+  require(blockSize == 64 || blockSize == 128 || blockSize == 256, "blockSize must be 64, 128 or 256");
+}
+int exa_renderer_set_launch_tuning(exa_renderer *r, int queueRounds, int queueDepth, int blockSize) {
+  return guarded([&] {
+    require(r, "renderer is NULL");
+    checkLaunchTuning(queueRounds, queueDepth, blockSize);
+    r->tuning.queueRounds = queueRounds;
+    r->tuning.queueDepth = queueDepth;
+    r->tuning.blockSize = blockSize;
+  });
+}
+int exa_renderer_get_tuning(const exa_renderer *r, int tuning[6]) {
+  if (!r || !tuning) return EXA_ERR_INVALID;
+  tuning[0] = r->tuning.ctrlSteps;
+  tuning[1] = r->tuning.elemBatch;
+  tuning[2] = r->tuning.rareBatch;
+  tuning[3] = r->tuning.queueRounds;
+  tuning[4] = r->tuning.queueDepth;
+  tuning[5] = r->tuning.blockSize;
+  return EXA_OK;
+}
 // ---- batched requests -------------------------------------------------------
 int exa_batch_create(exa_renderer *r, exa_batch **out) {
   return guarded([&] {
@@ -1522,6 +1595,23 @@ int exa_batch_get_tuning(const exa_batch *b, int tuning[3]) {
   tuning[2] = b->tuning.rareBatch;
   return EXA_OK;
 }
+int exa_batch_set_launch_tuning(exa_batch *b, int queueRounds, int queueDepth, int blockSize) {
+  return guarded([&] {
+    require(b, "batch is NULL");
+    checkLaunchTuning(queueRounds, queueDepth, blockSize);
+    b->tuning.queueRounds = queueRounds;
+    b->tuning.queueDepth = queueDepth;
+    b->tuning.blockSize = blockSize;
+    b->dropGraph();
+  });
+}
+int exa_batch_get_launch_tuning(const exa_batch *b, int tuning[3]) {
+  if (!b || !tuning) return EXA_ERR_INVALID;
+  tuning[0] = b->tuning.queueRounds;
+  tuning[1] = b->tuning.queueDepth;
+  tuning[2] = b->tuning.blockSize;
+  return EXA_OK;
+}
 int exa_batch_render(exa_batch *b, void *stream) {
   return guarded([&] {
     require(b, "batch is NULL");
@@ -1557,6 +1647,7 @@ int exa_batch_capture_graph(exa_batch *b, void *stream) {
     b->syncMajorants();
     b->dropGraph();
     hipStream_t st = (hipStream_t)stream;
+    if (b->tuning.queueRounds > 0) b->queue.view(b->numPixels, b->tuning.queueDepth);  // allocated before the capture
     HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     b->launch(nullptr, st);
     hipGraph_t g = nullptr;
@@ -1607,54 +1698,91 @@ int exa_batch_get_frame(const exa_batch *b, int req, uint32_t *out) {
 // Tries every combination of the candidate knob values on one frame each
 // (timed with HIP events on `stream`), keeps the fastest and resets the
 // accumulation (the trial frames accumulated).
+// One timed frame per combination of the candidate values of every axis;
+// keeps the fastest.  Axes: ctrlSteps, elemBatch, rareBatch, queueRounds,
+// queueDepth, blockSize.
+constexpr int kTuneAxes = 6;
+static PersistTuning autotuneBatch(exa_batch *b, const int *const axes[kTuneAxes], const int counts[kTuneAxes],
+                                   hipStream_t st, float *bestMs) {
+  for (int a = 0; a < kTuneAxes; ++a) require(axes[a] && counts[a] > 0, "every axis needs at least one candidate");
+  b->sync();
+  b->syncMajorants();
+  hipEvent_t e0, e1;
+  HIP_CHECK(hipEventCreate(&e0));
+  HIP_CHECK(hipEventCreate(&e1));
+  float bestT = 3.4e38f;
+  PersistTuning bestK = b->tuning;
+  b->launch(nullptr, st);  // warm-up
+  b->advanced();
+  int idx[kTuneAxes] = {};
+  for (;;) {
+    PersistTuning t;
+    t.ctrlSteps = axes[0][idx[0]];
+    t.elemBatch = axes[1][idx[1]];
+    t.rareBatch = axes[2][idx[2]];
+    t.queueRounds = axes[3][idx[3]];
+    t.queueDepth = axes[4][idx[4]];
+    t.blockSize = axes[5][idx[5]];
+    require(t.ctrlSteps >= 1 && t.ctrlSteps <= 1024 && t.elemBatch >= 0 && t.elemBatch <= 64 &&
+                t.rareBatch >= 0 && t.rareBatch <= 64,
+            "tuning candidate out of range");
+    checkLaunchTuning(t.queueRounds, t.queueDepth, t.blockSize);
+    b->tuning = t;
+    if (t.queueRounds > 0) b->queue.view(b->numPixels, t.queueDepth);  // allocation outside the timed frame
+    HIP_CHECK(hipEventRecord(e0, st));
+    b->launch(nullptr, st);
+    HIP_CHECK(hipEventRecord(e1, st));
+    HIP_CHECK(hipEventSynchronize(e1));
+    b->advanced();
+    float ms = 0.f;
+    HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    if (ms < bestT) {
+      bestT = ms;
+      bestK = t;
+    }
+    int a = 0;
+    while (a < kTuneAxes && ++idx[a] == counts[a]) idx[a++] = 0;
+    if (a == kTuneAxes) break;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  b->tuning = bestK;
+  b->dropGraph();
+  for (BatchRequest &q : b->host) q.P.accumID = 0;
+  b->dirty = true;
+  if (bestMs) *bestMs = bestT;
+  return bestK;
+}
+int exa_batch_autotune_launch(exa_batch *b, const int *ctrlSteps, int nCtrl, const int *elemBatch, int nElem,
+                              const int *rareBatch, int nRare, const int *queueRounds, int nRounds,
+                              const int *queueDepth, int nDepth, const int *blockSize, int nBlock, void *stream,
+                              int best[6], float *bestMs) {
+  return guarded([&] {
+    require(b && best, "invalid arguments");
+    const int *axes[kTuneAxes] = {ctrlSteps, elemBatch, rareBatch, queueRounds, queueDepth, blockSize};
+    const int counts[kTuneAxes] = {nCtrl, nElem, nRare, nRounds, nDepth, nBlock};
+    const PersistTuning k = autotuneBatch(b, axes, counts, (hipStream_t)stream, bestMs);
+    best[0] = k.ctrlSteps;
+    best[1] = k.elemBatch;
+    best[2] = k.rareBatch;
+    best[3] = k.queueRounds;
+    best[4] = k.queueDepth;
+    best[5] = k.blockSize;
+  });
+}
 int exa_batch_autotune(exa_batch *b, const int *ctrlSteps, int nCtrl, const int *elemBatch, int nElem,
                        const int *rareBatch, int nRare, void *stream, int best[3], float *bestMs) {
   return guarded([&] {
     require(b && ctrlSteps && elemBatch && rareBatch && nCtrl > 0 && nElem > 0 && nRare > 0 && best,
             "invalid arguments");
-    b->sync();
-    hipStream_t st = (hipStream_t)stream;
-    hipEvent_t e0, e1;
-    HIP_CHECK(hipEventCreate(&e0));
-    HIP_CHECK(hipEventCreate(&e1));
-    const PersistTuning saved = b->tuning;
-    float bestT = 3.4e38f;
-    PersistTuning bestK = saved;
-    b->launch(nullptr, st);  // warm-up
-    b->advanced();
-    for (int i = 0; i < nCtrl; ++i)
-      for (int j = 0; j < nElem; ++j)
-        for (int k = 0; k < nRare; ++k) {
-          PersistTuning t;
-          t.ctrlSteps = ctrlSteps[i];
-          t.elemBatch = elemBatch[j];
-          t.rareBatch = rareBatch[k];
-          require(t.ctrlSteps >= 1 && t.elemBatch >= 0 && t.elemBatch <= 64 && t.rareBatch >= 0 &&
-                      t.rareBatch <= 64,
-                  "tuning candidate out of range");
-          b->tuning = t;
-          HIP_CHECK(hipEventRecord(e0, st));
-          b->launch(nullptr, st);
-          HIP_CHECK(hipEventRecord(e1, st));
-          HIP_CHECK(hipEventSynchronize(e1));
-          b->advanced();
-          float ms = 0.f;
-          HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-          if (ms < bestT) {
-            bestT = ms;
-            bestK = t;
-          }
-        }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    b->tuning = bestK;
-    b->dropGraph();
-    for (BatchRequest &q : b->host) q.P.accumID = 0;
-    b->dirty = true;
-    best[0] = bestK.ctrlSteps;
-    best[1] = bestK.elemBatch;
-    best[2] = bestK.rareBatch;
-    if (bestMs) *bestMs = bestT;
+    // the launch shape stays as set
+    const int q = b->tuning.queueRounds, d = b->tuning.queueDepth, bs = b->tuning.blockSize;
+    const int *axes[kTuneAxes] = {ctrlSteps, elemBatch, rareBatch, &q, &d, &bs};
+    const int counts[kTuneAxes] = {nCtrl, nElem, nRare, 1, 1, 1};
+    const PersistTuning k = autotuneBatch(b, axes, counts, (hipStream_t)stream, bestMs);
+    best[0] = k.ctrlSteps;
+    best[1] = k.elemBatch;
+    best[2] = k.rareBatch;
   });
 }
 // ---- neural field sampler ---------------------------------------------------------

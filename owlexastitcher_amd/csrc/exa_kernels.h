@@ -174,6 +174,25 @@ struct PersistTuning {
   int32_t ctrlSteps = 6;   // tracking iterations per lane and round, >= 1
   int32_t elemBatch = 16;  // element pass when waiting lanes >= elemBatch/64 of live lanes
   int32_t rareBatch = 16;  // per-segment pass when waiting lanes >= rareBatch/64 of live lanes
+  int32_t queueRounds = 0; // element-stage queue rounds per frame (0: element pass in the renderer only)
+  int32_t queueDepth = 0;  // entries per element-stage queue (0: the frame's pixels, at most 2^21)
+  int32_t blockSize = 256; // threads per workgroup: 64, 128 or 256
+};
+constexpr int kMaxQueueRounds = 32;
+
+// Device buffers of the element-stage queue (two queues used in turn):
+// entries of kElementQueueVec float4 per path, a sample per entry, and the
+// counters (two queue counts, one hand-out counter).
+constexpr int kElementQueueVec = 16;
+struct ElementQueueBuffers {
+  float4 *rec[2];
+  Sample *samples[2];
+  uint32_t *keys[2];   // work id of each entry's path
+  uint32_t *order;     // launch order of a queue's entries (image order)
+  uint32_t *hist;      // maxBuckets counters of the ordering
+  uint32_t maxBuckets;
+  uint32_t *count;  // 3 uint32
+  uint32_t cap;     // entries per queue
 };
 // work: one device uint32 (reset by the launch itself)
 struct NeuralView;
@@ -181,13 +200,15 @@ struct NeuralView;
 // answer element-stage lookups with the neural field
 void launchRenderPersistent(const RenderParams &p, const SamplerView &sv, int accel, const GridView &gv,
                             uint32_t *work, const PersistTuning &t, unsigned long long *counters, hipStream_t s,
-                            const NeuralView *nv = nullptr, const uint8_t *netFlags = nullptr);
+                            const NeuralView *nv = nullptr, const uint8_t *netFlags = nullptr,
+                            const ElementQueueBuffers *qb = nullptr);
 // Batched launch over reqs[0..numReqs) (device array); numWork = total work
 // ids.  Afterwards every request's accumID is advanced by one on the device.
 // With nv, the network answers the element stage of the macrocells flagged in
 // netFlags (tables accelerator).
 void launchRenderBatch(const SamplerView &sv, int accel, const GridView &gv, BatchRequest *reqs, int numReqs,
                        uint32_t numWork, uint32_t *work, const PersistTuning &t, unsigned long long *counters,
-                       hipStream_t s, const NeuralView *nv = nullptr, const uint8_t *netFlags = nullptr);
+                       hipStream_t s, const NeuralView *nv = nullptr, const uint8_t *netFlags = nullptr,
+                       const ElementQueueBuffers *qb = nullptr);
 
 }  // namespace exa

@@ -27,11 +27,15 @@
 // the order of the megakernel (both use the helpers of exa_render_common.h),
 // so every pixel's result is bit-identical to it; only the interleaving of
 // lanes changes.
+#include <cstddef>
 #include "exa_render_common.h"
 #include "neural.h"
 
 namespace exa {
 
+// largest workgroup (the LDS arrays are sized for it); a launch may use 64,
+// 128 or 256 threads per workgroup (PersistTuning::blockSize): the waves of
+// a workgroup share nothing but the request table and the network weights
 constexpr int kPersistBlock = 256;
 // minimum waves per SIMD the register allocation must allow (spills beyond)
 #ifndef EXA_PERSIST_MIN_WAVES
@@ -46,6 +50,7 @@ enum Stage : int32_t {
   // point location and resolution
   kStAtSample,  // collision position ready for point location
   kStPending,   // gridlet stage missed: waits for the element pass
+  kStResolve,   // resumed from the element queue: its sample waits in xf
   // rare pass
   kStSegEnd,       // segment done: continue the integrator
   kStScatter,      // phase-function sample, next path ray
@@ -80,6 +85,45 @@ struct PathLane {
   float4 xf;
 };
 
+static_assert(sizeof(PathLane) <= kElementQueueVec * sizeof(float4), "a queued path fits its queue entry");
+constexpr int kParkVec = (int)((sizeof(PathLane) + sizeof(float4) - 1) / sizeof(float4));
+
+// Element-stage queue.  A lane whose gridlet stage missed writes its whole
+// path state into a queue entry and takes other work; a separate launch runs
+// the element stage over the entries with every lane busy, and the next
+// launch resumes the entries with their samples.  Entry i holds the path's
+// float4 k at rec[k * cap + i], so a wave's stores and loads of consecutive
+// entries are contiguous.
+struct ParkQueue {
+  float4 *rec;
+  Sample *samples;
+  uint32_t *keys;          // per entry: the path's work id (pixel tile order)
+  const uint32_t *order;   // entry i of the launch order -> stored entry (image order)
+  uint32_t *count;  // entries claimed; those at or beyond cap were not stored
+  uint32_t cap;
+};
+
+EXA_DEV void parkStore(const PathLane &L, const ParkQueue &q, uint32_t slot) {
+  float4 v[kParkVec];
+  __builtin_memcpy(v, &L, sizeof(PathLane));
+#pragma unroll
+  for (int k = 0; k < kParkVec; ++k) q.rec[(size_t)k * q.cap + slot] = v[k];
+}
+
+EXA_DEV void parkLoad(PathLane &L, const ParkQueue &q, uint32_t slot) {
+  float4 v[kParkVec];
+#pragma unroll
+  for (int k = 0; k < kParkVec; ++k) v[k] = q.rec[(size_t)k * q.cap + slot];
+  __builtin_memcpy(&L, v, sizeof(PathLane));
+}
+
+// one 32-bit word of a parked path
+EXA_DEV float parkWord(const ParkQueue &q, int word, uint32_t slot) {
+  const float4 v = q.rec[(size_t)(word >> 2) * q.cap + slot];
+  const int c = word & 3;
+  return c == 0 ? v.x : (c == 1 ? v.y : (c == 2 ? v.z : v.w));
+}
+
 struct PersistParams {
   NeuralView nv;              // neural field (kNeural launches)
   const uint8_t *netFlags;    // per macrocell: 1 = element stage answered by the network
@@ -91,6 +135,10 @@ struct PersistParams {
   int32_t ctrlSteps;  // tracking transitions per lane and round
   int32_t elemBatch;  // element pass runs when pending*64 >= live*elemBatch
   int32_t rareBatch;  // rare pass runs when rare*64 >= live*rareBatch
+  ParkQueue parkOut;    // kPark: element-stage entries written here
+  ParkQueue resumeIn;   // kResume: the work items (instead of pixels)
+  uint32_t *resumeNext; // kResume: hand-out counter over resumeIn
+  int32_t blockSize;    // threads per workgroup of the launch: 64, 128 or 256
 };
 
 EXA_DEV void beginSegment(PathLane &L, const RenderParams &P, const GridView &gv, const Ray &ray, int kind) {
@@ -296,7 +344,46 @@ EXA_DEV void grabPixels(PathLane &L, const RenderParams &P, const PersistParams 
   L.st = kStSampleStart;
 }
 
-template <bool kCount, int kAccel, bool kBatch, bool kNeural>
+// the work id a path's pixel was handed out under (8x8 tiles in scan order,
+// requests one after another): the key that orders the queue by image
+// position
+template <bool kBatch>
+EXA_DEV uint32_t workIdOf(const PathLane &L, const RenderParams &P, const PersistParams &pp,
+                          const uint32_t *workBegin) {
+  uint32_t tilesX = (uint32_t)((P.W + kPixelTile - 1) / kPixelTile), base = 0;
+  if (kBatch) {
+    tilesX = (uint32_t)pp.reqs[L.req].tilesX;
+    base = workBegin[L.req];
+  }
+  return base + ((((uint32_t)L.py >> 3) * tilesX + ((uint32_t)L.px >> 3)) << 6) + (((uint32_t)L.py & 7u) << 3) +
+         ((uint32_t)L.px & 7u);
+}
+
+// hand out queued paths to the lanes that wait for work (resume launches):
+// a lane takes the path and its element-stage sample, stashed in xf until
+// the resolve pass.  One atomic per wave.
+EXA_DEV void grabResumed(PathLane &L, const PersistParams &pp, int lane) {
+  const uint64_t want = __ballot(L.st == kStGrab);
+  if (!want) return;
+  const int leader = __ffsll((unsigned long long)want) - 1;
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(pp.resumeNext, (uint32_t)__popcll(want));
+  base = __shfl(base, leader);
+  if (L.st != kStGrab) return;
+  const uint32_t w = base + (uint32_t)__popcll(want & ((1ull << lane) - 1ull));
+  const uint32_t n = min(*pp.resumeIn.count, pp.resumeIn.cap);
+  if (w >= n) {
+    L.st = kStDone;
+    return;
+  }
+  const uint32_t slot = pp.resumeIn.order[w];
+  parkLoad(L, pp.resumeIn, slot);
+  const Sample s = pp.resumeIn.samples[slot];
+  L.xf = make_float4(__int_as_float(s.primID), __int_as_float(s.cellID), s.value, 0.f);
+  L.st = kStResolve;
+}
+
+template <bool kCount, int kAccel, bool kBatch, bool kNeural, bool kPark = false, bool kResume = false>
 __global__ __launch_bounds__(kPersistBlock) __attribute__((amdgpu_waves_per_eu(EXA_PERSIST_MIN_WAVES, 8))) void
 kRenderPersistent(RenderParams P, SamplerView sv, GridView gv, PersistParams pp, unsigned long long *counters) {
   __shared__ int32_t stackMem[kAccel == kAccelBvh ? kStackDepth * kPersistBlock : 1];
@@ -308,12 +395,12 @@ kRenderPersistent(RenderParams P, SamplerView sv, GridView gv, PersistParams pp,
   int32_t *stack = stackMem + threadIdx.x;
   const int lane = (int)__lane_id();
   if (kBatch) {
-    for (int i = threadIdx.x; i < pp.numReqs; i += kPersistBlock) workBegin[i] = pp.reqs[i].workBegin;
+    for (int i = threadIdx.x; i < pp.numReqs; i += blockDim.x) workBegin[i] = pp.reqs[i].workBegin;
     __syncthreads();
   }
   if (kNeural) {
-    for (int i = threadIdx.x; i < kNeuralIn * kNeuralHidden; i += kPersistBlock) nW1[i] = pp.nv.W1[i];
-    for (int i = threadIdx.x; i < kNeuralHidden * kNeuralHidden; i += kPersistBlock) nW2[i] = pp.nv.W2[i];
+    for (int i = threadIdx.x; i < kNeuralIn * kNeuralHidden; i += blockDim.x) nW1[i] = pp.nv.W1[i];
+    for (int i = threadIdx.x; i < kNeuralHidden * kNeuralHidden; i += blockDim.x) nW2[i] = pp.nv.W2[i];
     __syncthreads();
   }
   const int waveInBlock = (int)threadIdx.x >> 6;
@@ -329,7 +416,7 @@ kRenderPersistent(RenderParams P, SamplerView sv, GridView gv, PersistParams pp,
     if (!live) break;
     const int nLive = __popcll(live);
     // lanes that can progress without the gated passes
-    const bool stuck = __ballot(isTracking(L.st) || L.st == kStAtSample) == 0;
+    const bool stuck = __ballot(isTracking(L.st) || L.st == kStAtSample || L.st == kStResolve) == 0;
     // rare pass, in the order a lane flows through it
     const uint64_t rare = __ballot(isRare(L.st));
     if (rare && (stuck || __popcll(rare) * 64 >= nLive * pp.rareBatch)) {
@@ -342,7 +429,8 @@ kRenderPersistent(RenderParams P, SamplerView sv, GridView gv, PersistParams pp,
         if (L.st == kStSegEnd) segmentEnd<kCount>(L, Pr, kBatch ? R->numLights : numLights1, cnt, segRay, segKind);
         if (L.st == kStScatter) scatter(L, Pr);
       }
-      grabPixels<kBatch>(L, P, pp, workBegin, lane);
+      if (kResume) grabResumed(L, pp, lane);
+      else grabPixels<kBatch>(L, P, pp, workBegin, lane);
       // a grab may have moved the lane to another request (a lane that
       // begins a shadow segment does not grab)
       const BatchRequest *R = kBatch ? pp.reqs + L.req : nullptr;
@@ -390,6 +478,25 @@ kRenderPersistent(RenderParams P, SamplerView sv, GridView gv, PersistParams pp,
       }
     }
 
+    // element-stage queue: lanes whose gridlet stage missed leave their path
+    // in the queue and take other work (the lanes beyond its capacity stay
+    // for the element pass)
+    if (kPark) {
+      const uint64_t pm = __ballot(L.st == kStPending);
+      if (pm) {
+        const int leader = __ffsll((unsigned long long)pm) - 1;
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(pp.parkOut.count, (uint32_t)__popcll(pm));
+        base = __shfl(base, leader);
+        const uint32_t slot = base + (uint32_t)__popcll(pm & ((1ull << lane) - 1ull));
+        if (L.st == kStPending && slot < pp.parkOut.cap) {
+          parkStore(L, pp.parkOut, slot);
+          pp.parkOut.keys[slot] = workIdOf<kBatch>(L, Pl, pp, workBegin);
+          L.st = kStGrab;
+        }
+      }
+    }
+
     // element pass
     const uint64_t pend = __ballot(L.st == kStPending);
     if (pend && (stuck || __popcll(pend) * 64 >= nLive * pp.elemBatch)) {
@@ -415,6 +522,12 @@ kRenderPersistent(RenderParams P, SamplerView sv, GridView gv, PersistParams pp,
       }
     }
 
+    // a path resumed from the element queue brings its sample
+    if (kResume && L.st == kStResolve) {
+      smp = Sample{__float_as_int(L.xf.x), __float_as_int(L.xf.y), L.xf.z};
+      have = true;
+    }
+
     // resolve pass (Woodcock accept / ratio-tracking update)
     if (have) {
       if (kCount && smp.primID >= 0) cnt.c[kCntTfFetch]++;
@@ -436,29 +549,158 @@ __global__ void kResetWork(uint32_t *work) {
   if (threadIdx.x == 0) *work = 0u;
 }
 
+// zero the counters of one round of the element-stage queue (a kernel, not
+// a memset, so that a captured graph replays it)
+__global__ void kResetQueue(uint32_t *a, uint32_t *b, uint32_t *c) {
+  if (threadIdx.x == 0) {
+    if (a) *a = 0u;
+    if (b) *b = 0u;
+    if (c) *c = 0u;
+  }
+}
+
+// The element stage over the queued paths, one entry per lane: the point
+// location of locateTablesElements at the path's collision position, under
+// the culling of the path's transfer function.
+template <bool kCount, bool kBatch>
+__global__ __launch_bounds__(kLocateBlock) void kQueueElements(SamplerView sv, ParkQueue q,
+                                                                const BatchRequest *reqs,
+                                                                unsigned long long *counters) {
+  constexpr int wPos = (int)(offsetof(PathLane, pos) / sizeof(float));
+  constexpr int wReq = (int)(offsetof(PathLane, req) / sizeof(float));
+  const uint32_t n = min(*q.count, q.cap);
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    const uint32_t i = q.order[k];
+    const vec3f p = mk3(parkWord(q, wPos, i), parkWord(q, wPos + 1, i), parkWord(q, wPos + 2, i));
+    Culling cu = cullingOf(sv);
+    if (kBatch) {
+      const BatchRequest &R = reqs[__float_as_int(parkWord(q, wReq, i))];
+      cu.gridletMaxOp = R.gridletMaxOp;
+      cu.elemMaxOp = R.elemMaxOp;
+    }
+    int kind;
+    q.samples[i] = locateTablesElements(sv, cu, p, kind);
+    if (kCount) atomicAdd(&counters[kind < 0 ? kCntMiss : kCntGridletHit + kind], 1ull);
+  }
+}
+
 __global__ void kAdvanceBatch(BatchRequest *reqs, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) reqs[i].P.accumID += 1;
 }
 
-template <bool kCount, int kAccel, bool kBatch, bool kNeural>
-static int residentBlocks() {
-  static std::atomic<int> cache[kMaxCachedDevices];
-  return residentWorkgroups(kRenderPersistent<kCount, kAccel, kBatch, kNeural>, kPersistBlock, cache);
+// Image order of a queue: entries bucketed by work id >> shift (strips of
+// 2^shift pixels in tile order), a counting sort in three launches that read
+// the entry count on the device.  Order within a bucket is arbitrary.
+__global__ void kOrderCount(const uint32_t *keys, const uint32_t *count, uint32_t cap, int shift, uint32_t *hist) {
+  const uint32_t n = min(*count, cap);
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    atomicAdd(&hist[keys[i] >> shift], 1u);
 }
 
-template <bool kCount, int kAccel, bool kBatch, bool kNeural = false>
+__global__ __launch_bounds__(1024) void kOrderScan(uint32_t *hist, uint32_t nb) {
+  // exclusive scan of hist[0..nb) by one workgroup: per-thread chunk sums,
+  // a scan of the sums in LDS, then the chunks in place
+  __shared__ uint32_t part[1024];
+  const uint32_t t = threadIdx.x, per = (nb + 1023u) / 1024u;
+  const uint32_t b0 = min(nb, t * per), b1 = min(nb, b0 + per);
+  uint32_t sum = 0;
+  for (uint32_t b = b0; b < b1; ++b) sum += hist[b];
+  part[t] = sum;
+  __syncthreads();
+  for (uint32_t off = 1; off < 1024u; off <<= 1) {
+    const uint32_t v = t >= off ? part[t - off] : 0u;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  uint32_t run = part[t] - sum;
+  for (uint32_t b = b0; b < b1; ++b) {
+    const uint32_t c = hist[b];
+    hist[b] = run;
+    run += c;
+  }
+}
+
+__global__ void kOrderScatter(const uint32_t *keys, const uint32_t *count, uint32_t cap, int shift, uint32_t *offs,
+                              uint32_t *order) {
+  const uint32_t n = min(*count, cap);
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    order[atomicAdd(&offs[keys[i] >> shift], 1u)] = i;
+}
+
+__global__ void kZeroU32Array(uint32_t *p, uint32_t n) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0u;
+}
+
+template <bool kCount, int kAccel, bool kBatch, bool kNeural, bool kPark, bool kResume>
+static int residentBlocks() {
+  static std::atomic<int> cache[kMaxCachedDevices];
+  return residentWorkgroups(kRenderPersistent<kCount, kAccel, kBatch, kNeural, kPark, kResume>, kPersistBlock,
+                            cache);
+}
+
+// work items: pixels (64 per 8x8 tile) or, when resuming, at most the
+// queue's capacity
+template <bool kCount, int kAccel, bool kBatch, bool kNeural = false, bool kPark = false, bool kResume = false>
 static void launchPersistentT(const RenderParams &p, const SamplerView &sv, const GridView &gv,
                               const PersistParams &pp, unsigned long long *counters, hipStream_t s) {
-  const int blocks = std::max(1, std::min(residentBlocks<kCount, kAccel, kBatch, kNeural>(),
-                                          (int)((pp.numWork + kPersistBlock - 1) / kPersistBlock)));
-  hipLaunchKernelGGL((kRenderPersistent<kCount, kAccel, kBatch, kNeural>), dim3(blocks), dim3(kPersistBlock), 0, s,
-                     p, sv, gv, pp, counters);
+  const uint32_t items = kResume ? pp.resumeIn.cap : pp.numWork;
+  const int bs = pp.blockSize;
+  // the resident waves (occupancy is set by registers, not by workgroup size)
+  const int64_t waves = (int64_t)residentBlocks<kCount, kAccel, kBatch, kNeural, kPark, kResume>() * (kPersistBlock / 64);
+  const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>(waves / (bs / 64), ((int64_t)items + bs - 1) / bs));
+  hipLaunchKernelGGL((kRenderPersistent<kCount, kAccel, kBatch, kNeural, kPark, kResume>), dim3(blocks), dim3(bs),
+                     0, s, p, sv, gv, pp, counters);
+}
+
+// A frame with the element-stage queue: the first launch renders the pixels
+// and queues every element-stage lookup; then, per round, the element stage
+// runs over the queue and the next launch resumes the queued paths, queueing
+// their next element-stage lookups into the other buffer; the last launch
+// resumes without queueing (its lanes run the element pass themselves), so
+// every path ends within rounds + 1 launches.  No host synchronisation.
+template <bool kCount, bool kBatch>
+static void launchQueued(const RenderParams &p, const SamplerView &sv, const GridView &gv, PersistParams pp,
+                         unsigned long long *counters, hipStream_t s, const ElementQueueBuffers &qb, int rounds) {
+  ParkQueue q[2];
+  for (int k = 0; k < 2; ++k) q[k] = ParkQueue{qb.rec[k], qb.samples[k], qb.keys[k], qb.order, qb.count + k, qb.cap};
+  uint32_t *resumeNext = qb.count + 2;
+  // buckets of 2^shift work ids, at most qb.maxBuckets of them
+  int shift = 6;
+  while (((uint64_t)pp.numWork >> shift) + 1 > qb.maxBuckets) ++shift;
+  const uint32_t nb = (uint32_t)(((uint64_t)pp.numWork >> shift) + 1);
+  hipLaunchKernelGGL(kResetQueue, dim3(1), dim3(64), 0, s, pp.work, q[0].count, resumeNext);
+  pp.parkOut = q[0];
+  pp.resumeIn = q[1];
+  pp.resumeNext = resumeNext;
+  launchPersistentT<kCount, kAccelTables, kBatch, false, true, false>(p, sv, gv, pp, counters, s);
+  const dim3 eg(std::max(1u, std::min(4096u, (qb.cap + kLocateBlock - 1) / kLocateBlock)));
+  for (int r = 1; r <= rounds; ++r) {
+    const ParkQueue in = q[(r - 1) & 1], out = q[r & 1];
+    hipLaunchKernelGGL(kZeroU32Array, dim3(std::min(1024u, (nb + 255) / 256)), dim3(256), 0, s, qb.hist, nb);
+    hipLaunchKernelGGL(kOrderCount, eg, dim3(kLocateBlock), 0, s, in.keys, in.count, in.cap, shift, qb.hist);
+    hipLaunchKernelGGL(kOrderScan, dim3(1), dim3(1024), 0, s, qb.hist, nb);
+    hipLaunchKernelGGL(kOrderScatter, eg, dim3(kLocateBlock), 0, s, in.keys, in.count, in.cap, shift, qb.hist,
+                       qb.order);
+    hipLaunchKernelGGL((kQueueElements<kCount, kBatch>), eg, dim3(kLocateBlock), 0, s, sv, in, pp.reqs, counters);
+    hipLaunchKernelGGL(kResetQueue, dim3(1), dim3(64), 0, s, out.count, resumeNext, nullptr);
+    pp.resumeIn = in;
+    pp.parkOut = out;
+    if (r < rounds) launchPersistentT<kCount, kAccelTables, kBatch, false, true, true>(p, sv, gv, pp, counters, s);
+    else launchPersistentT<kCount, kAccelTables, kBatch, false, false, true>(p, sv, gv, pp, counters, s);
+  }
 }
 
 template <bool kBatch>
 static void launchAny(const RenderParams &p, const SamplerView &sv, int accel, const GridView &gv,
-                      const PersistParams &pp, unsigned long long *counters, hipStream_t s) {
+                      const PersistParams &pp, unsigned long long *counters, hipStream_t s,
+                      const ElementQueueBuffers *qb, int rounds) {
+  if (qb && qb->cap > 0 && rounds > 0 && !pp.netFlags && accel == kAccelTables) {
+    if (counters) launchQueued<true, kBatch>(p, sv, gv, pp, counters, s, *qb, rounds);
+    else launchQueued<false, kBatch>(p, sv, gv, pp, counters, s, *qb, rounds);
+    return;
+  }
   hipLaunchKernelGGL(kResetWork, dim3(1), dim3(64), 0, s, pp.work);
   if (pp.netFlags) {  // neural field: tables accelerator
     if (counters) launchPersistentT<true, kAccelTables, kBatch, true>(p, sv, gv, pp, counters, s);
@@ -478,11 +720,19 @@ static void setTuning(PersistParams &pp, const PersistTuning &t) {
   pp.ctrlSteps = t.ctrlSteps < 1 ? 1 : t.ctrlSteps;
   pp.elemBatch = t.elemBatch < 0 ? 0 : (t.elemBatch > 64 ? 64 : t.elemBatch);
   pp.rareBatch = t.rareBatch < 0 ? 0 : (t.rareBatch > 64 ? 64 : t.rareBatch);
+  pp.parkOut = ParkQueue{};
+  pp.resumeIn = ParkQueue{};
+  pp.resumeNext = nullptr;
+  pp.blockSize = (t.blockSize == 64 || t.blockSize == 128) ? t.blockSize : kPersistBlock;
+}
+
+static int queueRounds(const PersistTuning &t) {
+  return t.queueRounds < 0 ? 0 : (t.queueRounds > kMaxQueueRounds ? kMaxQueueRounds : t.queueRounds);
 }
 
 void launchRenderPersistent(const RenderParams &p, const SamplerView &sv, int accel, const GridView &gv,
                             uint32_t *work, const PersistTuning &t, unsigned long long *counters, hipStream_t s,
-                            const NeuralView *nv, const uint8_t *netFlags) {
+                            const NeuralView *nv, const uint8_t *netFlags, const ElementQueueBuffers *qb) {
   PersistParams pp;
   pp.nv = NeuralView{};
   if (nv) pp.nv = *nv;
@@ -494,12 +744,12 @@ void launchRenderPersistent(const RenderParams &p, const SamplerView &sv, int ac
   const uint32_t tilesY = (uint32_t)((p.H + kPixelTile - 1) / kPixelTile);
   pp.numWork = (uint32_t)pp.tilesX * tilesY * 64u;
   setTuning(pp, t);
-  launchAny<false>(p, sv, accel, gv, pp, counters, s);
+  launchAny<false>(p, sv, accel, gv, pp, counters, s, qb, queueRounds(t));
 }
 
 void launchRenderBatch(const SamplerView &sv, int accel, const GridView &gv, BatchRequest *reqs, int numReqs,
                        uint32_t numWork, uint32_t *work, const PersistTuning &t, unsigned long long *counters,
-                       hipStream_t s, const NeuralView *nv, const uint8_t *netFlags) {
+                       hipStream_t s, const NeuralView *nv, const uint8_t *netFlags, const ElementQueueBuffers *qb) {
   PersistParams pp;
   pp.nv = nv ? *nv : NeuralView{};
   pp.netFlags = nv ? netFlags : nullptr;
@@ -510,7 +760,7 @@ void launchRenderBatch(const SamplerView &sv, int accel, const GridView &gv, Bat
   pp.numWork = numWork;
   setTuning(pp, t);
   const RenderParams unused{};  // every lane reads its request's parameters
-  launchAny<true>(unused, sv, accel, gv, pp, counters, s);
+  launchAny<true>(unused, sv, accel, gv, pp, counters, s, qb, queueRounds(t));
   hipLaunchKernelGGL(kAdvanceBatch, dim3((numReqs + 63) / 64), dim3(64), 0, s, reqs, numReqs);
 }
 

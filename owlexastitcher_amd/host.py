@@ -562,6 +562,19 @@ class OWLRenderer:
         """Scheduling knobs of RENDER_PERSISTENT; images do not depend on them."""
         check(lib().exa_renderer_set_persistent_tuning(self.h, int(ctrlSteps), int(elemBatch), int(rareBatch)))
 
+    def setLaunchTuning(self, queue_rounds=0, queue_depth=0, block_size=256):
+        """Launch shape of RENDER_PERSISTENT: element-stage queue rounds per
+        frame (0: none), entries per queue (0: one per pixel, at most 2^21)
+        and threads per workgroup; images do not depend on it."""
+        check(lib().exa_renderer_set_launch_tuning(self.h, int(queue_rounds), int(queue_depth), int(block_size)))
+
+    @property
+    def tuning(self):
+        """(ctrlSteps, elemBatch, rareBatch, queueRounds, queueDepth, blockSize)"""
+        t = np.zeros(6, np.int32)
+        check(lib().exa_renderer_get_tuning(self.h, _ptr(t)))
+        return tuple(int(x) for x in t)
+
     def setDebugCrosshair(self, on):
         check(lib().exa_renderer_set_debug_crosshair(self.h, int(bool(on))))
 
@@ -779,6 +792,16 @@ class RenderBatch:
         check(lib().exa_batch_get_tuning(self.h, _ptr(t)))
         return tuple(int(x) for x in t)
 
+    def setLaunchTuning(self, queue_rounds=0, queue_depth=0, block_size=256):
+        check(lib().exa_batch_set_launch_tuning(self.h, int(queue_rounds), int(queue_depth), int(block_size)))
+
+    @property
+    def launchTuning(self):
+        """(queueRounds, queueDepth, blockSize)"""
+        t = np.zeros(3, np.int32)
+        check(lib().exa_batch_get_launch_tuning(self.h, _ptr(t)))
+        return tuple(int(x) for x in t)
+
     def render(self, stream=None):
         check(lib().exa_batch_render(self.h, _stream(stream)))
 
@@ -821,6 +844,21 @@ class RenderBatch:
         ms = C.c_float()
         check(lib().exa_batch_autotune(self.h, _ptr(c), len(c), _ptr(e), len(e), _ptr(r), len(r),
                                        _stream(stream), _ptr(best), C.byref(ms)))
+        return tuple(int(x) for x in best), ms.value
+
+    def autotuneLaunch(self, ctrl_steps=(6,), elem_batch=(16,), rare_batch=(16,), queue_rounds=(0, 4, 8),
+                       queue_depth=(0,), block_size=(64, 128, 256), stream=None):
+        """The autotuner over six axes: the scheduling knobs, the element-stage
+        queue rounds and depth, and the workgroup size.  Returns
+        ((ctrlSteps, elemBatch, rareBatch, queueRounds, queueDepth, blockSize), best_ms)."""
+        ax = [np.ascontiguousarray(a, np.int32) for a in (ctrl_steps, elem_batch, rare_batch, queue_rounds,
+                                                          queue_depth, block_size)]
+        best = np.zeros(6, np.int32)
+        ms = C.c_float()
+        args = []
+        for a in ax:
+            args += [_ptr(a), len(a)]
+        check(lib().exa_batch_autotune_launch(self.h, *args, _stream(stream), _ptr(best), C.byref(ms)))
         return tuple(int(x) for x in best), ms.value
 
 

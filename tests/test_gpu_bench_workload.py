@@ -65,12 +65,15 @@ def workload(amr):
     return r, om, maj, rgba
 
 
-@pytest.mark.parametrize("scheme", [RENDER_PERSISTENT, RENDER_MEGAKERNEL], ids=["persistent", "megakernel"])
-def test_timed_frame_matches_oracle(workload, device, scheme):
+@pytest.mark.parametrize("scheme,rounds", [(RENDER_PERSISTENT, 0), (RENDER_MEGAKERNEL, 0), (RENDER_PERSISTENT, 8)],
+                         ids=["persistent", "megakernel", "persistent-queue8"])
+def test_timed_frame_matches_oracle(workload, device, scheme, rounds):
     import torch
     r, om, maj, rgba = workload
-    print(f"case: timed workload 1920x1080, scheme {scheme}, 2 frames", flush=True)
+    print(f"case: timed workload 1920x1080, scheme {scheme}, element-stage queue rounds {rounds}, 2 frames",
+          flush=True)
     r.setScheme(scheme)
+    r.setLaunchTuning(rounds, 0, 256)
     r.resetAccum()
     fb = torch.zeros((H, W), dtype=torch.int32, device=device)
     acc_o = None
@@ -85,6 +88,7 @@ def test_timed_frame_matches_oracle(workload, device, scheme):
     assert np.array_equal(fb.cpu().numpy().view(np.uint32), fb_o)
     # the view shows the volume, not only the background
     assert np.count_nonzero(np.any(acc != acc[:1, :1], axis=-1)) > W * H // 10
+    r.setLaunchTuning(0, 0, 256)
 
 
 def test_locate_workload_matches_oracle(workload, device):
