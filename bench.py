@@ -81,7 +81,7 @@ def setup_renderer(model, args):
     from owlexastitcher_amd import RENDER_MEGAKERNEL, RENDER_PERSISTENT
     r.setScheme(RENDER_PERSISTENT if args.scheme == "persistent" else RENDER_MEGAKERNEL)
     r.setPersistentTuning(args.ctrl_steps, args.elem_batch, args.rare_batch)
-    r.setLaunchTuning(args.queue_rounds, args.queue_depth, args.block_size)
+    r.setLaunchTuning(args.element_stage, args.queue_rounds, args.queue_depth, args.block_size)
     if args.accel == "bvh":
         from owlexastitcher_amd import ACCEL_BVH
         t = time.time()
@@ -711,7 +711,9 @@ def main():
     ap.add_argument("--ctrl-steps", type=int, default=6)
     ap.add_argument("--elem-batch", type=int, default=16)
     ap.add_argument("--rare-batch", type=int, default=16)
-    ap.add_argument("--queue-rounds", type=int, default=0, help="element-stage queue rounds per frame (0: none)")
+    ap.add_argument("--element-stage", type=int, default=0, choices=[0, 1, 2],
+                    help="0: in the waiting lanes' wave; 1: workgroup queue in LDS; 2: device queue between launches")
+    ap.add_argument("--queue-rounds", type=int, default=1, help="element stage 2: queue rounds per frame")
     ap.add_argument("--queue-depth", type=int, default=0, help="entries per element-stage queue (0: one per pixel)")
     ap.add_argument("--block-size", type=int, default=256, choices=[64, 128, 256])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -829,8 +831,8 @@ def main():
             "accel": args.accel,
             "scheme": args.scheme,
             "persistent_tuning": [args.ctrl_steps, args.elem_batch, args.rare_batch],
-            "launch_tuning": {"queue_rounds": args.queue_rounds, "queue_depth": args.queue_depth,
-                              "block_size": args.block_size},
+            "launch_tuning": {"element_stage": args.element_stage, "queue_rounds": args.queue_rounds,
+                              "queue_depth": args.queue_depth, "block_size": args.block_size},
             "frame_ms": kernel_ms,
             "volume_samples_per_s": queries / (kernel_ms * 1e-3),
             "counters": cnt,

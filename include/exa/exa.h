@@ -282,20 +282,32 @@ int exa_renderer_set_accel(exa_renderer *r, int accel);
 int exa_renderer_set_scheme(exa_renderer *r, int scheme);
 int exa_renderer_get_scheme(const exa_renderer *r);
 int exa_renderer_set_persistent_tuning(exa_renderer *r, int ctrlSteps, int elemBatch, int rareBatch);
-/* Launch shape of the persistent scheme (no reference counterpart; results
-   do not depend on it).  queueRounds in [0, 32]: rounds of the element-stage
-   queue per frame: lanes whose gridlet stage misses leave their path in a
-   queue and take other work; a separate launch runs the element stage over
-   the queue with every lane busy; the next launch resumes the queued paths;
-   the last of rounds + 1 launches runs the element stage in its own lanes
-   (0 = no queue, the element pass of the renderer only; the table
-   accelerator without a neural field).  queueDepth in [0, 2^24]: entries
-   per queue (0 = one per pixel, at most 2^21); a path that finds the queue
-   full runs the renderer's element pass.  blockSize: 64, 128 or 256 threads
-   per workgroup.  Defaults 0, 0, 256. */
-int exa_renderer_set_launch_tuning(exa_renderer *r, int queueRounds, int queueDepth, int blockSize);
-/* ctrlSteps, elemBatch, rareBatch, queueRounds, queueDepth, blockSize */
-int exa_renderer_get_tuning(const exa_renderer *r, int tuning[6]);
+/* Launch shape of the persistent scheme (no reference counterpart; images
+   do not depend on it).  elementStage: where a path's element-stage lookup
+   runs when its gridlet stage misses:
+   EXA_ELEMENT_IN_WAVE (default): in its own wave, once elemBatch/64 of the
+     wave's live lanes wait for it;
+   EXA_ELEMENT_WORKGROUP_QUEUE: the lane posts the query to its workgroup's
+     queue in LDS and waits; any wave of the workgroup answers up to 64
+     posted queries at once when elemBatch (>= 1) of them are posted, or
+     when all its own live lanes wait;
+   EXA_ELEMENT_QUEUE_ROUNDS: the lane leaves its whole path in a device
+     queue and takes other work; between launches the element stage runs over
+     the queue with every lane busy and the next launch resumes the paths;
+     queueRounds in [1, 32] such rounds per frame, the last launch running
+     the element stage in its own lanes; queueDepth in [0, 2^24] entries per
+     queue (0 = one per pixel, at most 2^21; paths that find it full run the
+     element stage in their wave).
+   The queues need the table accelerator without a neural field (otherwise
+   the element stage runs in the wave).  blockSize: 64, 128 or 256 threads
+   per workgroup.  Defaults: in wave, 1, 0, 256. */
+#define EXA_ELEMENT_IN_WAVE 0
+#define EXA_ELEMENT_WORKGROUP_QUEUE 1
+#define EXA_ELEMENT_QUEUE_ROUNDS 2
+int exa_renderer_set_launch_tuning(exa_renderer *r, int elementStage, int queueRounds, int queueDepth,
+                                   int blockSize);
+/* ctrlSteps, elemBatch, rareBatch, elementStage, queueRounds, queueDepth, blockSize */
+int exa_renderer_get_tuning(const exa_renderer *r, int tuning[7]);
 
 /* Snapshot of the launch parameters of the next render() call (for checking
    against the CPU oracle). */
@@ -354,11 +366,11 @@ int exa_batch_add_request(exa_batch *b, const exa_batch_request *req, int *index
 int exa_batch_num_requests(const exa_batch *b);
 int exa_batch_set_tuning(exa_batch *b, int ctrlSteps, int elemBatch, int rareBatch);
 int exa_batch_get_tuning(const exa_batch *b, int tuning[3]);
-/* queue rounds, queue depth and workgroup size as
+/* element stage, queue rounds, queue depth and workgroup size as
    exa_renderer_set_launch_tuning (depth 0: one entry per pixel of all
    requests, at most 2^21) */
-int exa_batch_set_launch_tuning(exa_batch *b, int queueRounds, int queueDepth, int blockSize);
-int exa_batch_get_launch_tuning(const exa_batch *b, int tuning[3]);
+int exa_batch_set_launch_tuning(exa_batch *b, int elementStage, int queueRounds, int queueDepth, int blockSize);
+int exa_batch_get_launch_tuning(const exa_batch *b, int tuning[4]);
 /* one frame of every request (asynchronous on stream); accumIDs advance */
 int exa_batch_render(exa_batch *b, void *stream);
 int exa_batch_render_counted(exa_batch *b, uint64_t counters[11], void *stream);
@@ -376,12 +388,13 @@ int exa_batch_get_frame(const exa_batch *b, int req, uint32_t *hostOut);  /* RGB
    accumulation. */
 int exa_batch_autotune(exa_batch *b, const int *ctrlSteps, int nCtrl, const int *elemBatch, int nElem,
                        const int *rareBatch, int nRare, void *stream, int best[3], float *bestMs);
-/* the same over six axes: the three scheduling knobs, the element-stage
-   queue rounds and queue depth, and the workgroup size; best[6] */
+/* the same over seven axes: the three scheduling knobs, where the element
+   stage runs, the queue rounds and queue depth, and the workgroup size;
+   best[7] */
 int exa_batch_autotune_launch(exa_batch *b, const int *ctrlSteps, int nCtrl, const int *elemBatch, int nElem,
-                              const int *rareBatch, int nRare, const int *queueRounds, int nRounds,
-                              const int *queueDepth, int nDepth, const int *blockSize, int nBlock, void *stream,
-                              int best[6], float *bestMs);
+                              const int *rareBatch, int nRare, const int *elementStage, int nStage,
+                              const int *queueRounds, int nRounds, const int *queueDepth, int nDepth,
+                              const int *blockSize, int nBlock, void *stream, int best[7], float *bestMs);
 
 /* ---- neural field sampler (build-side; no reference counterpart) ----
    A multiresolution hash encoding (levels x 2^log2T entries x 2 bf16

@@ -7,7 +7,7 @@ raises ImportError if it has not been built.
 """
 from ._capi import LIB_PATH, ExaError, lib  # noqa: F401
 from .host import (ACCEL_BVH, ACCEL_TABLES, DIRECT_LIGHTING, PATH_TRACER, RAY_MARCHER, RENDER_MEGAKERNEL,  # noqa: F401
-                   RENDER_PERSISTENT, AmrScene, DualMesh, ExaBrickModel, ExaBrickSampler, ExaStitchModel,
+                   ELEMENT_IN_WAVE, ELEMENT_QUEUE_ROUNDS, ELEMENT_WORKGROUP_QUEUE, RENDER_PERSISTENT, AmrScene, DualMesh, ExaBrickModel, ExaBrickSampler, ExaStitchModel,
                    ExaStitchSampler, NeuralField, OWLRenderer,
                    RenderBatch,
                    camera_look_at, make_grids,

@@ -160,7 +160,7 @@ _SIGS = {
     "exa_renderer_set_scheme": (C.c_int, [P, C.c_int]),
     "exa_renderer_get_scheme": (C.c_int, [P]),
     "exa_renderer_set_persistent_tuning": (C.c_int, [P, C.c_int, C.c_int, C.c_int]),
-    "exa_renderer_set_launch_tuning": (C.c_int, [P, C.c_int, C.c_int, C.c_int]),
+    "exa_renderer_set_launch_tuning": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_int]),
     "exa_renderer_get_tuning": (C.c_int, [P, P]),
     "exa_renderer_get_state": (C.c_int, [P, P]),
     "exa_renderer_destroy": (None, [P]),
@@ -170,7 +170,7 @@ _SIGS = {
     "exa_batch_add_request": (C.c_int, [P, P, P]),
     "exa_batch_num_requests": (C.c_int, [P]),
     "exa_batch_set_tuning": (C.c_int, [P, C.c_int, C.c_int, C.c_int]),
-    "exa_batch_set_launch_tuning": (C.c_int, [P, C.c_int, C.c_int, C.c_int]),
+    "exa_batch_set_launch_tuning": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_int]),
     "exa_batch_get_launch_tuning": (C.c_int, [P, P]),
     "exa_batch_get_tuning": (C.c_int, [P, P]),
     "exa_batch_render": (C.c_int, [P, P]),
@@ -183,7 +183,7 @@ _SIGS = {
     "exa_batch_get_frame": (C.c_int, [P, C.c_int, P]),
     "exa_batch_autotune": (C.c_int, [P, P, C.c_int, P, C.c_int, P, C.c_int, P, P, P]),
     "exa_batch_autotune_launch": (C.c_int, [P, P, C.c_int, P, C.c_int, P, C.c_int, P, C.c_int, P, C.c_int, P,
-                                            C.c_int, P, P, P]),
+                                            C.c_int, P, C.c_int, P, P, P]),
     "exa_neural_create": (C.c_int, [P, C.c_int, C.c_int, F, F, C.c_uint32, P]),
     "exa_neural_destroy": (None, [P]),
     "exa_neural_get_config": (C.c_int, [P, P, P]),

@@ -650,7 +650,7 @@ struct exa_batch {
     NeuralView nv{};
     if (n) nv = n->view();
     ElementQueueBuffers qb{};
-    const bool queued = tuning.queueRounds > 0 && !n && r->sampler.accel == kAccelTables;
+    const bool queued = tuning.elementStage == kElemQueueRounds && !n && r->sampler.accel == kAccelTables;
     if (queued) qb = queue.view(numPixels, tuning.queueDepth);
     launchRenderBatch(r->sampler.view(), r->sampler.accel, r->sampler.gridView(), dev.ptr, (int)host.size(),
                       numWork, work.ptr, tuning, counters, s, n ? &nv : nullptr, n ? n->flags.ptr : nullptr,
@@ -1237,7 +1237,7 @@ static void doRender(exa_renderer *r, uint32_t *fb, unsigned long long *counters
     NeuralView nv{};
     if (useNet) nv = r->neural->view();
     ElementQueueBuffers qb{};
-    const bool queued = r->tuning.queueRounds > 0 && !useNet && r->sampler.accel == kAccelTables;
+    const bool queued = r->tuning.elementStage == kElemQueueRounds && !useNet && r->sampler.accel == kAccelTables;
     if (queued) qb = r->queue.view((int64_t)P.W * P.H, r->tuning.queueDepth);
     launchRenderPersistent(P, r->sampler.view(), r->sampler.accel, r->sampler.gridView(), r->work.ptr, r->tuning,
                            counters, s, useNet ? &nv : nullptr, useNet ? r->neural->flags.ptr : nullptr,
@@ -1458,28 +1458,36 @@ int exa_renderer_set_persistent_tuning(exa_renderer *r, int ctrlSteps, int elemB
     r->tuning.rareBatch = rareBatch;
   });
 }
-static void checkLaunchTuning(int queueRounds, int queueDepth, int blockSize) {
-  require(queueRounds >= 0 && queueRounds <= kMaxQueueRounds, "queueRounds must be in [0, 32]");
+static void checkLaunchTuning(int elementStage, int queueRounds, int queueDepth, int blockSize) {
+  require(elementStage >= EXA_ELEMENT_IN_WAVE && elementStage <= EXA_ELEMENT_QUEUE_ROUNDS,
+          "elementStage must be EXA_ELEMENT_IN_WAVE, EXA_ELEMENT_WORKGROUP_QUEUE or EXA_ELEMENT_QUEUE_ROUNDS");
+  require(queueRounds >= 1 && queueRounds <= kMaxQueueRounds, "queueRounds must be in [1, 32]");
   require(queueDepth >= 0 && queueDepth <= (1 << 24), "queueDepth must be in [0, 2^24]");
   require(blockSize == 64 || blockSize == 128 || blockSize == 256, "blockSize must be 64, 128 or 256");
 }
-int exa_renderer_set_launch_tuning(exa_renderer *r, int queueRounds, int queueDepth, int blockSize) {
+static void setLaunchTuning(PersistTuning &t, int elementStage, int queueRounds, int queueDepth, int blockSize) {
+  checkLaunchTuning(elementStage, queueRounds, queueDepth, blockSize);
+  t.elementStage = elementStage;
+  t.queueRounds = queueRounds;
+  t.queueDepth = queueDepth;
+  t.blockSize = blockSize;
+}
+int exa_renderer_set_launch_tuning(exa_renderer *r, int elementStage, int queueRounds, int queueDepth,
+                                   int blockSize) {
   return guarded([&] {
     require(r, "renderer is NULL");
-    checkLaunchTuning(queueRounds, queueDepth, blockSize);
-    r->tuning.queueRounds = queueRounds;
-    r->tuning.queueDepth = queueDepth;
-    r->tuning.blockSize = blockSize;
+    setLaunchTuning(r->tuning, elementStage, queueRounds, queueDepth, blockSize);
   });
 }
-int exa_renderer_get_tuning(const exa_renderer *r, int tuning[6]) {
+int exa_renderer_get_tuning(const exa_renderer *r, int tuning[7]) {
   if (!r || !tuning) return EXA_ERR_INVALID;
   tuning[0] = r->tuning.ctrlSteps;
   tuning[1] = r->tuning.elemBatch;
   tuning[2] = r->tuning.rareBatch;
-  tuning[3] = r->tuning.queueRounds;
-  tuning[4] = r->tuning.queueDepth;
-  tuning[5] = r->tuning.blockSize;
+  tuning[3] = r->tuning.elementStage;
+  tuning[4] = r->tuning.queueRounds;
+  tuning[5] = r->tuning.queueDepth;
+  tuning[6] = r->tuning.blockSize;
   return EXA_OK;
 }
 // ---- batched requests -------------------------------------------------------
@@ -1595,21 +1603,19 @@ int exa_batch_get_tuning(const exa_batch *b, int tuning[3]) {
   tuning[2] = b->tuning.rareBatch;
   return EXA_OK;
 }
-int exa_batch_set_launch_tuning(exa_batch *b, int queueRounds, int queueDepth, int blockSize) {
+int exa_batch_set_launch_tuning(exa_batch *b, int elementStage, int queueRounds, int queueDepth, int blockSize) {
   return guarded([&] {
     require(b, "batch is NULL");
-    checkLaunchTuning(queueRounds, queueDepth, blockSize);
-    b->tuning.queueRounds = queueRounds;
-    b->tuning.queueDepth = queueDepth;
-    b->tuning.blockSize = blockSize;
+    setLaunchTuning(b->tuning, elementStage, queueRounds, queueDepth, blockSize);
     b->dropGraph();
   });
 }
-int exa_batch_get_launch_tuning(const exa_batch *b, int tuning[3]) {
+int exa_batch_get_launch_tuning(const exa_batch *b, int tuning[4]) {
   if (!b || !tuning) return EXA_ERR_INVALID;
-  tuning[0] = b->tuning.queueRounds;
-  tuning[1] = b->tuning.queueDepth;
-  tuning[2] = b->tuning.blockSize;
+  tuning[0] = b->tuning.elementStage;
+  tuning[1] = b->tuning.queueRounds;
+  tuning[2] = b->tuning.queueDepth;
+  tuning[3] = b->tuning.blockSize;
   return EXA_OK;
 }
 int exa_batch_render(exa_batch *b, void *stream) {
@@ -1647,7 +1653,7 @@ int exa_batch_capture_graph(exa_batch *b, void *stream) {
     b->syncMajorants();
     b->dropGraph();
     hipStream_t st = (hipStream_t)stream;
-    if (b->tuning.queueRounds > 0) b->queue.view(b->numPixels, b->tuning.queueDepth);  // allocated before the capture
+    if (b->tuning.elementStage == kElemQueueRounds) b->queue.view(b->numPixels, b->tuning.queueDepth);  // before the capture
     HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     b->launch(nullptr, st);
     hipGraph_t g = nullptr;
@@ -1699,9 +1705,9 @@ int exa_batch_get_frame(const exa_batch *b, int req, uint32_t *out) {
 // (timed with HIP events on `stream`), keeps the fastest and resets the
 // accumulation (the trial frames accumulated).
 // One timed frame per combination of the candidate values of every axis;
-// keeps the fastest.  Axes: ctrlSteps, elemBatch, rareBatch, queueRounds,
-// queueDepth, blockSize.
-constexpr int kTuneAxes = 6;
+// keeps the fastest.  Axes: ctrlSteps, elemBatch, rareBatch, elementStage,
+// queueRounds, queueDepth, blockSize.
+constexpr int kTuneAxes = 7;
 static PersistTuning autotuneBatch(exa_batch *b, const int *const axes[kTuneAxes], const int counts[kTuneAxes],
                                    hipStream_t st, float *bestMs) {
   for (int a = 0; a < kTuneAxes; ++a) require(axes[a] && counts[a] > 0, "every axis needs at least one candidate");
@@ -1720,15 +1726,12 @@ static PersistTuning autotuneBatch(exa_batch *b, const int *const axes[kTuneAxes
     t.ctrlSteps = axes[0][idx[0]];
     t.elemBatch = axes[1][idx[1]];
     t.rareBatch = axes[2][idx[2]];
-    t.queueRounds = axes[3][idx[3]];
-    t.queueDepth = axes[4][idx[4]];
-    t.blockSize = axes[5][idx[5]];
     require(t.ctrlSteps >= 1 && t.ctrlSteps <= 1024 && t.elemBatch >= 0 && t.elemBatch <= 64 &&
                 t.rareBatch >= 0 && t.rareBatch <= 64,
             "tuning candidate out of range");
-    checkLaunchTuning(t.queueRounds, t.queueDepth, t.blockSize);
+    setLaunchTuning(t, axes[3][idx[3]], axes[4][idx[4]], axes[5][idx[5]], axes[6][idx[6]]);
     b->tuning = t;
-    if (t.queueRounds > 0) b->queue.view(b->numPixels, t.queueDepth);  // allocation outside the timed frame
+    if (t.elementStage == kElemQueueRounds) b->queue.view(b->numPixels, t.queueDepth);  // outside the timed frame
     HIP_CHECK(hipEventRecord(e0, st));
     b->launch(nullptr, st);
     HIP_CHECK(hipEventRecord(e1, st));
@@ -1754,20 +1757,21 @@ static PersistTuning autotuneBatch(exa_batch *b, const int *const axes[kTuneAxes
   return bestK;
 }
 int exa_batch_autotune_launch(exa_batch *b, const int *ctrlSteps, int nCtrl, const int *elemBatch, int nElem,
-                              const int *rareBatch, int nRare, const int *queueRounds, int nRounds,
-                              const int *queueDepth, int nDepth, const int *blockSize, int nBlock, void *stream,
-                              int best[6], float *bestMs) {
+                              const int *rareBatch, int nRare, const int *elementStage, int nStage,
+                              const int *queueRounds, int nRounds, const int *queueDepth, int nDepth,
+                              const int *blockSize, int nBlock, void *stream, int best[7], float *bestMs) {
   return guarded([&] {
     require(b && best, "invalid arguments");
-    const int *axes[kTuneAxes] = {ctrlSteps, elemBatch, rareBatch, queueRounds, queueDepth, blockSize};
-    const int counts[kTuneAxes] = {nCtrl, nElem, nRare, nRounds, nDepth, nBlock};
+    const int *axes[kTuneAxes] = {ctrlSteps, elemBatch, rareBatch, elementStage, queueRounds, queueDepth, blockSize};
+    const int counts[kTuneAxes] = {nCtrl, nElem, nRare, nStage, nRounds, nDepth, nBlock};
     const PersistTuning k = autotuneBatch(b, axes, counts, (hipStream_t)stream, bestMs);
     best[0] = k.ctrlSteps;
     best[1] = k.elemBatch;
     best[2] = k.rareBatch;
-    best[3] = k.queueRounds;
-    best[4] = k.queueDepth;
-    best[5] = k.blockSize;
+    best[3] = k.elementStage;
+    best[4] = k.queueRounds;
+    best[5] = k.queueDepth;
+    best[6] = k.blockSize;
   });
 }
 int exa_batch_autotune(exa_batch *b, const int *ctrlSteps, int nCtrl, const int *elemBatch, int nElem,
@@ -1776,9 +1780,10 @@ int exa_batch_autotune(exa_batch *b, const int *ctrlSteps, int nCtrl, const int 
     require(b && ctrlSteps && elemBatch && rareBatch && nCtrl > 0 && nElem > 0 && nRare > 0 && best,
             "invalid arguments");
     // the launch shape stays as set
-    const int q = b->tuning.queueRounds, d = b->tuning.queueDepth, bs = b->tuning.blockSize;
-    const int *axes[kTuneAxes] = {ctrlSteps, elemBatch, rareBatch, &q, &d, &bs};
-    const int counts[kTuneAxes] = {nCtrl, nElem, nRare, 1, 1, 1};
+    const int e = b->tuning.elementStage, q = b->tuning.queueRounds, d = b->tuning.queueDepth,
+              bs = b->tuning.blockSize;
+    const int *axes[kTuneAxes] = {ctrlSteps, elemBatch, rareBatch, &e, &q, &d, &bs};
+    const int counts[kTuneAxes] = {nCtrl, nElem, nRare, 1, 1, 1, 1};
     const PersistTuning k = autotuneBatch(b, axes, counts, (hipStream_t)stream, bestMs);
     best[0] = k.ctrlSteps;
     best[1] = k.elemBatch;

@@ -174,10 +174,17 @@ struct PersistTuning {
   int32_t ctrlSteps = 6;   // tracking iterations per lane and round, >= 1
   int32_t elemBatch = 16;  // element pass when waiting lanes >= elemBatch/64 of live lanes
   int32_t rareBatch = 16;  // per-segment pass when waiting lanes >= rareBatch/64 of live lanes
-  int32_t queueRounds = 0; // element-stage queue rounds per frame (0: element pass in the renderer only)
-  int32_t queueDepth = 0;  // entries per element-stage queue (0: the frame's pixels, at most 2^21)
+  // where the element stage runs: kElemInWave (the waiting lanes' own wave,
+  // gated by elemBatch), kElemWorkgroupQueue (any wave of the workgroup
+  // answers up to 64 queries posted in LDS, once elemBatch are posted),
+  // kElemQueueRounds (queue between launches: queueRounds rounds of
+  // queueDepth entries, 0 = one per pixel)
+  int32_t elementStage = 0;
+  int32_t queueRounds = 1;
+  int32_t queueDepth = 0;
   int32_t blockSize = 256; // threads per workgroup: 64, 128 or 256
 };
+enum ElementStage { kElemInWave = 0, kElemWorkgroupQueue = 1, kElemQueueRounds = 2 };
 constexpr int kMaxQueueRounds = 32;
 
 // Device buffers of the element-stage queue (two queues used in turn):

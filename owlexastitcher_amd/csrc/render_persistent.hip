@@ -51,6 +51,7 @@ enum Stage : int32_t {
   kStAtSample,  // collision position ready for point location
   kStPending,   // gridlet stage missed: waits for the element pass
   kStResolve,   // resumed from the element queue: its sample waits in xf
+  kStWaitElem,  // element-stage query posted to the workgroup's queue
   // rare pass
   kStSegEnd,       // segment done: continue the integrator
   kStScatter,      // phase-function sample, next path ray
@@ -139,6 +140,7 @@ struct PersistParams {
   ParkQueue resumeIn;   // kResume: the work items (instead of pixels)
   uint32_t *resumeNext; // kResume: hand-out counter over resumeIn
   int32_t blockSize;    // threads per workgroup of the launch: 64, 128 or 256
+  int32_t sharedQueue;  // 1: element stage through the workgroup's LDS queue
 };
 
 EXA_DEV void beginSegment(PathLane &L, const RenderParams &P, const GridView &gv, const Ray &ray, int kind) {
@@ -383,9 +385,29 @@ EXA_DEV void grabResumed(PathLane &L, const PersistParams &pp, int lane) {
   L.st = kStResolve;
 }
 
-template <bool kCount, int kAccel, bool kBatch, bool kNeural, bool kPark = false, bool kResume = false>
+// Workgroup element queue in LDS (kShared): a lane whose gridlet stage
+// missed posts its query (position, request) in its own slot and sets its
+// bit in `posted`; a wave of the workgroup claims up to 64 posted queries at
+// once (atomic AND on the bit words) and answers them with every lane busy,
+// writing each result to the query's slot and setting its bit in `done`;
+// the posting lane picks its result up at its next pass.  No barriers: a
+// wave whose live lanes all wait answers whatever is posted itself, and
+// claimed queries are answered at once, so every query is answered while its
+// lane's wave lives.
+template <bool kOn>
+struct ElementQueueLds {
+  static constexpr int N = kOn ? kPersistBlock : 1;
+  float4 pos[N];   // x, y, z, request (int bits)
+  uint2 res[N];    // primID, value bits
+  unsigned long long posted[kPersistBlock / 64], done[kPersistBlock / 64];
+  uint16_t claim[kPersistBlock / 64][kOn ? 64 : 1];  // per wave: the slots it claimed
+};
+
+template <bool kCount, int kAccel, bool kBatch, bool kNeural, bool kPark = false, bool kResume = false,
+          bool kShared = false>
 __global__ __launch_bounds__(kPersistBlock) __attribute__((amdgpu_waves_per_eu(EXA_PERSIST_MIN_WAVES, 8))) void
 kRenderPersistent(RenderParams P, SamplerView sv, GridView gv, PersistParams pp, unsigned long long *counters) {
+  __shared__ ElementQueueLds<kShared> eq;
   __shared__ int32_t stackMem[kAccel == kAccelBvh ? kStackDepth * kPersistBlock : 1];
   __shared__ uint32_t workBegin[kBatch ? kMaxBatchRequests : 1];
   // neural field: bf16 weights, and per wave the encodings and first hidden layer
@@ -404,6 +426,14 @@ kRenderPersistent(RenderParams P, SamplerView sv, GridView gv, PersistParams pp,
     __syncthreads();
   }
   const int waveInBlock = (int)threadIdx.x >> 6;
+  if (kShared) {
+    if (threadIdx.x < kPersistBlock / 64) {
+      eq.posted[threadIdx.x] = 0ull;
+      eq.done[threadIdx.x] = 0ull;
+    }
+    __syncthreads();
+  }
+  const int numWords = (int)(blockDim.x >> 6);
   LocalCounters cnt;
   if (kCount)
     for (int i = 0; i < kNumCounters; ++i) cnt.c[i] = 0;
@@ -497,9 +527,81 @@ kRenderPersistent(RenderParams P, SamplerView sv, GridView gv, PersistParams pp,
       }
     }
 
+    if (kShared) {
+      const int me = (int)threadIdx.x;
+      // post: each waiting lane's query in its own slot, one atomic per wave
+      const uint64_t pm = __ballot(L.st == kStPending);
+      if (pm) {
+        if (L.st == kStPending) {
+          eq.pos[me] = make_float4(L.pos.x, L.pos.y, L.pos.z, __int_as_float(L.req));
+          L.st = kStWaitElem;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        if (lane == __ffsll((unsigned long long)pm) - 1) atomicOr(&eq.posted[waveInBlock], (unsigned long long)pm);
+      }
+      // answer a batch of posted queries: when enough are posted, or when
+      // every live lane of this wave waits for an answer
+      const bool waiting = __ballot(L.st != kStWaitElem && L.st != kStDone) == 0;
+      int total = 0;
+      for (int w = 0; w < numWords; ++w) total += __popcll(__hip_atomic_load(&eq.posted[w], __ATOMIC_RELAXED,
+                                                                             __HIP_MEMORY_SCOPE_WORKGROUP));
+      if (total > 0 && (waiting || total >= (pp.elemBatch > 0 ? pp.elemBatch : 1))) {
+        // claim up to 64 slots, word by word (bits another wave took first
+        // are not in the returned word)
+        int nClaim = 0;
+        for (int w = 0; w < numWords && nClaim < 64; ++w) {
+          unsigned long long want = 0ull;
+          if (lane == 0) {
+            const unsigned long long m = __hip_atomic_load(&eq.posted[w], __ATOMIC_RELAXED,
+                                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+            unsigned long long take = m;
+            for (int k = __popcll(m); k > 64 - nClaim; --k) take &= ~(1ull << (63 - __clzll(take)));
+            want = take ? (atomicAnd(&eq.posted[w], ~take) & take) : 0ull;
+          }
+          const unsigned long long got = __shfl(want, 0);
+          if (got & (1ull << lane))
+            eq.claim[waveInBlock][nClaim + __popcll(got & ((1ull << lane) - 1ull))] = (uint16_t)(w * 64 + lane);
+          nClaim += __popcll(got);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+        if (lane < nClaim) {
+          const int slot = eq.claim[waveInBlock][lane];
+          const float4 q = eq.pos[slot];
+          Culling cq = cullingOf(sv);
+          if (kBatch) {
+            const BatchRequest &Rq = pp.reqs[__float_as_int(q.w)];
+            cq.gridletMaxOp = Rq.gridletMaxOp;
+            cq.elemMaxOp = Rq.elemMaxOp;
+          }
+          int kind;
+          const Sample a = locateTablesElements(sv, cq, mk3(q.x, q.y, q.z), kind);
+          if (kCount) cnt.c[kind < 0 ? kCntMiss : kCntGridletHit + kind]++;
+          eq.res[slot] = make_uint2((uint32_t)a.primID, __float_as_uint(a.value));
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+          atomicOr(&eq.done[slot >> 6], 1ull << (slot & 63));
+        }
+      } else if (waiting && total == 0) {
+        __builtin_amdgcn_s_sleep(1);  // the answers are being computed by another wave
+      }
+      // pick up answers
+      if (L.st == kStWaitElem) {
+        const unsigned long long d = __hip_atomic_load(&eq.done[waveInBlock], __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (d & (1ull << lane)) {
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+          const uint2 r = eq.res[me];
+          smp = Sample{(int32_t)r.x, -1, __uint_as_float(r.y)};
+          have = true;
+          L.st = kStPending;  // resolved below
+        }
+      }
+      const uint64_t taken = __ballot(have && L.st == kStPending);
+      if (taken && lane == __ffsll((unsigned long long)taken) - 1) atomicAnd(&eq.done[waveInBlock], ~(unsigned long long)taken);
+    }
+
     // element pass
-    const uint64_t pend = __ballot(L.st == kStPending);
-    if (pend && (stuck || __popcll(pend) * 64 >= nLive * pp.elemBatch)) {
+    const uint64_t pend = __ballot(L.st == kStPending && !have);
+    if (!kShared && pend && (stuck || __popcll(pend) * 64 >= nLive * pp.elemBatch)) {
       bool net = false;
       if (kNeural) {
         // validated macrocells: the neural field answers (whole wave, MFMA)
@@ -633,25 +735,27 @@ __global__ void kZeroU32Array(uint32_t *p, uint32_t n) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0u;
 }
 
-template <bool kCount, int kAccel, bool kBatch, bool kNeural, bool kPark, bool kResume>
+template <bool kCount, int kAccel, bool kBatch, bool kNeural, bool kPark, bool kResume, bool kShared>
 static int residentBlocks() {
   static std::atomic<int> cache[kMaxCachedDevices];
-  return residentWorkgroups(kRenderPersistent<kCount, kAccel, kBatch, kNeural, kPark, kResume>, kPersistBlock,
-                            cache);
+  return residentWorkgroups(kRenderPersistent<kCount, kAccel, kBatch, kNeural, kPark, kResume, kShared>,
+                            kPersistBlock, cache);
 }
 
 // work items: pixels (64 per 8x8 tile) or, when resuming, at most the
 // queue's capacity
-template <bool kCount, int kAccel, bool kBatch, bool kNeural = false, bool kPark = false, bool kResume = false>
+template <bool kCount, int kAccel, bool kBatch, bool kNeural = false, bool kPark = false, bool kResume = false,
+          bool kShared = false>
 static void launchPersistentT(const RenderParams &p, const SamplerView &sv, const GridView &gv,
                               const PersistParams &pp, unsigned long long *counters, hipStream_t s) {
   const uint32_t items = kResume ? pp.resumeIn.cap : pp.numWork;
   const int bs = pp.blockSize;
   // the resident waves (occupancy is set by registers, not by workgroup size)
-  const int64_t waves = (int64_t)residentBlocks<kCount, kAccel, kBatch, kNeural, kPark, kResume>() * (kPersistBlock / 64);
+  const int64_t waves =
+      (int64_t)residentBlocks<kCount, kAccel, kBatch, kNeural, kPark, kResume, kShared>() * (kPersistBlock / 64);
   const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>(waves / (bs / 64), ((int64_t)items + bs - 1) / bs));
-  hipLaunchKernelGGL((kRenderPersistent<kCount, kAccel, kBatch, kNeural, kPark, kResume>), dim3(blocks), dim3(bs),
-                     0, s, p, sv, gv, pp, counters);
+  hipLaunchKernelGGL((kRenderPersistent<kCount, kAccel, kBatch, kNeural, kPark, kResume, kShared>), dim3(blocks),
+                     dim3(bs), 0, s, p, sv, gv, pp, counters);
 }
 
 // A frame with the element-stage queue: the first launch renders the pixels
@@ -702,6 +806,11 @@ static void launchAny(const RenderParams &p, const SamplerView &sv, int accel, c
     return;
   }
   hipLaunchKernelGGL(kResetWork, dim3(1), dim3(64), 0, s, pp.work);
+  if (pp.sharedQueue && !pp.netFlags && accel == kAccelTables) {
+    if (counters) launchPersistentT<true, kAccelTables, kBatch, false, false, false, true>(p, sv, gv, pp, counters, s);
+    else launchPersistentT<false, kAccelTables, kBatch, false, false, false, true>(p, sv, gv, pp, counters, s);
+    return;
+  }
   if (pp.netFlags) {  // neural field: tables accelerator
     if (counters) launchPersistentT<true, kAccelTables, kBatch, true>(p, sv, gv, pp, counters, s);
     else launchPersistentT<false, kAccelTables, kBatch, true>(p, sv, gv, pp, counters, s);
@@ -724,10 +833,12 @@ static void setTuning(PersistParams &pp, const PersistTuning &t) {
   pp.resumeIn = ParkQueue{};
   pp.resumeNext = nullptr;
   pp.blockSize = (t.blockSize == 64 || t.blockSize == 128) ? t.blockSize : kPersistBlock;
+  pp.sharedQueue = t.elementStage == kElemWorkgroupQueue ? 1 : 0;
 }
 
 static int queueRounds(const PersistTuning &t) {
-  return t.queueRounds < 0 ? 0 : (t.queueRounds > kMaxQueueRounds ? kMaxQueueRounds : t.queueRounds);
+  if (t.elementStage != kElemQueueRounds) return 0;
+  return t.queueRounds < 1 ? 1 : (t.queueRounds > kMaxQueueRounds ? kMaxQueueRounds : t.queueRounds);
 }
 
 void launchRenderPersistent(const RenderParams &p, const SamplerView &sv, int accel, const GridView &gv,

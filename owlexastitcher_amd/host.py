@@ -24,6 +24,9 @@ ACCEL_TABLES = 0
 ACCEL_BVH = 1
 RENDER_MEGAKERNEL = 0
 RENDER_PERSISTENT = 1
+ELEMENT_IN_WAVE = 0
+ELEMENT_WORKGROUP_QUEUE = 1
+ELEMENT_QUEUE_ROUNDS = 2
 
 
 def _ptr(a):
@@ -562,16 +565,18 @@ class OWLRenderer:
         """Scheduling knobs of RENDER_PERSISTENT; images do not depend on them."""
         check(lib().exa_renderer_set_persistent_tuning(self.h, int(ctrlSteps), int(elemBatch), int(rareBatch)))
 
-    def setLaunchTuning(self, queue_rounds=0, queue_depth=0, block_size=256):
-        """Launch shape of RENDER_PERSISTENT: element-stage queue rounds per
-        frame (0: none), entries per queue (0: one per pixel, at most 2^21)
-        and threads per workgroup; images do not depend on it."""
-        check(lib().exa_renderer_set_launch_tuning(self.h, int(queue_rounds), int(queue_depth), int(block_size)))
+    def setLaunchTuning(self, element_stage=ELEMENT_IN_WAVE, queue_rounds=1, queue_depth=0, block_size=256):
+        """Launch shape of RENDER_PERSISTENT: where the element stage runs
+        (ELEMENT_IN_WAVE, ELEMENT_WORKGROUP_QUEUE, ELEMENT_QUEUE_ROUNDS), the
+        queue rounds per frame and entries per queue of the last (0: one per
+        pixel), threads per workgroup; images do not depend on it."""
+        check(lib().exa_renderer_set_launch_tuning(self.h, int(element_stage), int(queue_rounds), int(queue_depth),
+                                                   int(block_size)))
 
     @property
     def tuning(self):
-        """(ctrlSteps, elemBatch, rareBatch, queueRounds, queueDepth, blockSize)"""
-        t = np.zeros(6, np.int32)
+        """(ctrlSteps, elemBatch, rareBatch, elementStage, queueRounds, queueDepth, blockSize)"""
+        t = np.zeros(7, np.int32)
         check(lib().exa_renderer_get_tuning(self.h, _ptr(t)))
         return tuple(int(x) for x in t)
 
@@ -792,13 +797,14 @@ class RenderBatch:
         check(lib().exa_batch_get_tuning(self.h, _ptr(t)))
         return tuple(int(x) for x in t)
 
-    def setLaunchTuning(self, queue_rounds=0, queue_depth=0, block_size=256):
-        check(lib().exa_batch_set_launch_tuning(self.h, int(queue_rounds), int(queue_depth), int(block_size)))
+    def setLaunchTuning(self, element_stage=0, queue_rounds=1, queue_depth=0, block_size=256):
+        check(lib().exa_batch_set_launch_tuning(self.h, int(element_stage), int(queue_rounds), int(queue_depth),
+                                                int(block_size)))
 
     @property
     def launchTuning(self):
-        """(queueRounds, queueDepth, blockSize)"""
-        t = np.zeros(3, np.int32)
+        """(elementStage, queueRounds, queueDepth, blockSize)"""
+        t = np.zeros(4, np.int32)
         check(lib().exa_batch_get_launch_tuning(self.h, _ptr(t)))
         return tuple(int(x) for x in t)
 
@@ -846,14 +852,15 @@ class RenderBatch:
                                        _stream(stream), _ptr(best), C.byref(ms)))
         return tuple(int(x) for x in best), ms.value
 
-    def autotuneLaunch(self, ctrl_steps=(6,), elem_batch=(16,), rare_batch=(16,), queue_rounds=(0, 4, 8),
-                       queue_depth=(0,), block_size=(64, 128, 256), stream=None):
-        """The autotuner over six axes: the scheduling knobs, the element-stage
-        queue rounds and depth, and the workgroup size.  Returns
-        ((ctrlSteps, elemBatch, rareBatch, queueRounds, queueDepth, blockSize), best_ms)."""
-        ax = [np.ascontiguousarray(a, np.int32) for a in (ctrl_steps, elem_batch, rare_batch, queue_rounds,
-                                                          queue_depth, block_size)]
-        best = np.zeros(6, np.int32)
+    def autotuneLaunch(self, ctrl_steps=(6,), elem_batch=(16,), rare_batch=(16,), element_stage=(0, 1),
+                       queue_rounds=(1,), queue_depth=(0,), block_size=(64, 128, 256), stream=None):
+        """The autotuner over seven axes: the scheduling knobs, where the
+        element stage runs, the queue rounds and depth, and the workgroup
+        size.  Returns ((ctrlSteps, elemBatch, rareBatch, elementStage,
+        queueRounds, queueDepth, blockSize), best_ms)."""
+        ax = [np.ascontiguousarray(a, np.int32) for a in (ctrl_steps, elem_batch, rare_batch, element_stage,
+                                                          queue_rounds, queue_depth, block_size)]
+        best = np.zeros(7, np.int32)
         ms = C.c_float()
         args = []
         for a in ax:

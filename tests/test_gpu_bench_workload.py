@@ -65,15 +65,15 @@ def workload(amr):
     return r, om, maj, rgba
 
 
-@pytest.mark.parametrize("scheme,rounds", [(RENDER_PERSISTENT, 0), (RENDER_MEGAKERNEL, 0), (RENDER_PERSISTENT, 8)],
-                         ids=["persistent", "megakernel", "persistent-queue8"])
-def test_timed_frame_matches_oracle(workload, device, scheme, rounds):
+@pytest.mark.parametrize("scheme,stage", [(RENDER_PERSISTENT, 0), (RENDER_MEGAKERNEL, 0), (RENDER_PERSISTENT, 1),
+                                          (RENDER_PERSISTENT, 2)],
+                         ids=["persistent", "megakernel", "persistent-wgqueue", "persistent-rounds"])
+def test_timed_frame_matches_oracle(workload, device, scheme, stage):
     import torch
     r, om, maj, rgba = workload
-    print(f"case: timed workload 1920x1080, scheme {scheme}, element-stage queue rounds {rounds}, 2 frames",
-          flush=True)
+    print(f"case: timed workload 1920x1080, scheme {scheme}, element stage {stage}, 2 frames", flush=True)
     r.setScheme(scheme)
-    r.setLaunchTuning(rounds, 0, 256)
+    r.setLaunchTuning(stage, 4, 0, 256)
     r.resetAccum()
     fb = torch.zeros((H, W), dtype=torch.int32, device=device)
     acc_o = None
@@ -88,7 +88,7 @@ def test_timed_frame_matches_oracle(workload, device, scheme, rounds):
     assert np.array_equal(fb.cpu().numpy().view(np.uint32), fb_o)
     # the view shows the volume, not only the background
     assert np.count_nonzero(np.any(acc != acc[:1, :1], axis=-1)) > W * H // 10
-    r.setLaunchTuning(0, 0, 256)
+    r.setLaunchTuning(0, 1, 0, 256)
 
 
 def test_locate_workload_matches_oracle(workload, device):
