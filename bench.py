@@ -295,11 +295,15 @@ def batch_workload(r, args, dev):
 
     t = time.time()
     tune = make(1)
-    best, best_ms = tune.autotune((8, 16, 32), (8, 16, 32), (0, 8, 16))
+    # seven axes: ctrlSteps, elemBatch, rareBatch, element stage (in the wave /
+    # workgroup queue in LDS / device queue rounds), queue rounds, queue depth,
+    # workgroup size
+    best, best_ms = tune.autotuneLaunch((8, 16), (16, 32), (0, 8), (0, 1, 2), (2,), (0, 1 << 18), (128, 256))
     log(f"[bench] autotune on the 1-spp batch: best {best} ({best_ms:.1f} ms/frame), {time.time() - t:.1f}s")
     del tune
     bt = make(args.batch_spp)
-    bt.setTuning(*best)
+    bt.setTuning(*best[:3])
+    bt.setLaunchTuning(*best[3:])
     stream = torch.cuda.current_stream()
     for _ in range(args.warmup):
         bt.render(stream)

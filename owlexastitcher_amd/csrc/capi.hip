@@ -479,6 +479,7 @@ struct exa_neural {
     }
     nv.vmin = vmin;
     nv.vrange = vrange;
+    nv.widened = nullptr;
     return nv;
   }
   void pack(hipStream_t s) {
@@ -648,7 +649,10 @@ struct exa_batch {
   void launch(unsigned long long *counters, hipStream_t s) {
     const exa_neural *n = activeNeural();
     NeuralView nv{};
-    if (n) nv = n->view();
+    if (n) {
+      nv = n->view();
+      nv.widened = n->widened.ptr;
+    }
     ElementQueueBuffers qb{};
     const bool queued = tuning.elementStage == kElemQueueRounds && !n && r->sampler.accel == kAccelTables;
     if (queued) qb = queue.view(numPixels, tuning.queueDepth);
@@ -1235,7 +1239,10 @@ static void doRender(exa_renderer *r, uint32_t *fb, unsigned long long *counters
     if (useNet && r->sampler.accel != kAccelTables)
       throw Status(EXA_ERR_STATE, "the neural field runs with the table accelerator");
     NeuralView nv{};
-    if (useNet) nv = r->neural->view();
+    if (useNet) {
+      nv = r->neural->view();
+      nv.widened = r->neural->widened.ptr;
+    }
     ElementQueueBuffers qb{};
     const bool queued = r->tuning.elementStage == kElemQueueRounds && !useNet && r->sampler.accel == kAccelTables;
     if (queued) qb = r->queue.view((int64_t)P.W * P.H, r->tuning.queueDepth);

@@ -29,6 +29,10 @@ struct NeuralView {
   const uint16_t *W1, *W2, *w3;       // bf16, row-major [in][out]
   float lo[3], invExtent[3];          // x01 = (p - lo) * invExtent
   float vmin, vrange;                 // value = vmin + y * vrange
+  // renderer: per macrocell the exact value range widened by the validation
+  // bound; the network's value is clamped into its macrocell's range, so the
+  // majorant built over that range bounds it (nullptr: raw values)
+  const float2 *widened;
 };
 
 // ---- bf16 ----------------------------------------------------------------

@@ -605,11 +605,16 @@ kRenderPersistent(RenderParams P, SamplerView sv, GridView gv, PersistParams pp,
       bool net = false;
       if (kNeural) {
         // validated macrocells: the neural field answers (whole wave, MFMA)
-        net = L.st == kStPending && pp.netFlags[segmentCellID(gvl, L.S)];
+        const int64_t mc = segmentCellID(gvl, L.S);
+        net = L.st == kStPending && pp.netFlags[mc];
         if (__ballot(net)) {
-          const float v = neuralValue(pp.nv, L.pos, nW1, nW2, nX + waveInBlock * 64 * kNeuralIn,
-                                      nH1 + waveInBlock * 64 * kNeuralHidden);
+          float v = neuralValue(pp.nv, L.pos, nW1, nW2, nX + waveInBlock * 64 * kNeuralIn,
+                                nH1 + waveInBlock * 64 * kNeuralHidden);
           if (net) {
+            if (pp.nv.widened) {
+              const float2 w = pp.nv.widened[mc];
+              if (w.x <= w.y) v = fminf(fmaxf(v, w.x), w.y);
+            }
             smp = Sample{0x7fffffff, -1, v};
             if (kCount) cnt.c[kCntNeural]++;
             have = true;

@@ -257,3 +257,76 @@ def test_bricks_workload_full_size(amr, workload, device):
     assert np.array_equal(f0, f1)
     assert np.count_nonzero(np.any(a0 != a0[:1, :1], axis=-1)) > W * H // 10
 
+
+
+def test_learned_sampler_full_size(workload, device):
+    """The learned-sampler configuration at its full size on the timed scene:
+    hash grid of 16 levels x 2^19 entries x 2 features (bf16, float32 master
+    weights), MLP 32-64-64-1, a pool of 2^24 samples drawn by the exact
+    sampler (70% in stitching elements), 20 000 Adam steps of 2^18 samples
+    (LR 1e-2 decaying to 1e-4); per-macrocell validation at 1e-3 of the
+    value range; then 1080p frames with the network answering the element
+    stage of the validated macrocells against frames of the exact sampler
+    with the same seeds.  Reports what the network gains (frame time, image
+    error); asserts the training, the validation bound, the widened majorants
+    and the image error."""
+    import torch
+    from owlexastitcher_amd import NeuralField
+    r, om, maj, rgba = workload
+    print("case: learned sampler, full size (16 x 2^19 x 2 grid, 2^24 pool, 20000 x 2^18 steps)", flush=True)
+    # the timed frame's settings (earlier tests of this file change the view)
+    b = r.modelBounds
+    center = 0.5 * (b[:3] + b[3:])
+    r.resize(W, H)
+    r.setCamera(*camera_look_at(center + np.array([-0.3, 0.7, 1.0]) * (b[3:] - b[:3]), center, (0, 1, 0), 70.0, W,
+                                H))
+    r.setColorMap(rgba)
+    vr = r.valueRange
+    r.setRange(vr[0], vr[1])
+    r.setOpacityScale(100.0)
+    r.setType(0)
+    r.setSpp(1)
+    r.setScheme(RENDER_PERSISTENT)
+    r.setLaunchTuning(0, 1, 0, 256)
+    f = NeuralField(r, levels=16, log2T=19, n_min=16, n_max=512, seed=1)
+    assert f.makePool(1 << 24, 0.7, seed=7) > 0.9 * (1 << 24)
+    loss = f.train(20000, batch=1 << 18, lr=1e-2, lr_end=1e-4)
+    print(f"  loss {loss[:5].mean():.3g} -> {loss[-50:].mean():.3g}", flush=True)
+    assert np.all(np.isfinite(loss)) and loss[-50:].mean() < 1e-3 * loss[:5].mean()
+    v = f.validate(points_per_mc=64, bound=1e-3 * float(vr[1] - vr[0]))
+    print(f"  validation {v}", flush=True)
+    assert v["neural_mcs"] > 0 and v["max_error"] <= v["bound"]
+    K = 8
+    fb = torch.zeros((H, W), dtype=torch.int32, device=device)
+    stream = torch.cuda.current_stream()
+
+    def frames(k):
+        r.accumID = 0
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for _ in range(k):
+            r.render(fb, stream)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / k, r.accum().astype(np.float64)
+
+    r.setNeural(None)
+    ms_exact, a1 = frames(K)
+    for _ in range(K):
+        r.render(fb, stream)
+    torch.cuda.synchronize()
+    a2 = r.accum().astype(np.float64)
+    r.setNeural(f)
+    ms_net, n1 = frames(K)
+    r.accumID = 0
+    c = r.render_counted(fb, stream)
+    r.setNeural(None)
+    A, B, N = a1 / K, (a2 - a1) / K, n1 / K
+    noise = float(np.sqrt(np.mean((B - A)[..., :3] ** 2)))
+    err = float(np.sqrt(np.mean((N - A)[..., :3] ** 2)))
+    print(f"  exact {ms_exact:.2f} ms/frame, with the network {ms_net:.2f} ms/frame "
+          f"(x{ms_exact / ms_net:.3f}); image RMSE vs exact {err:.3g} (exact seed noise {noise:.3g}); "
+          f"network lookups {c['neural']}, exact element lookups {c['pyr'] + c['wedge'] + c['hex']}", flush=True)
+    assert c["neural"] > 0
+    assert np.all(np.isfinite(N)) and err <= noise

@@ -3,11 +3,15 @@ forward values, loss and every gradient (bf16 MFMA with float32
 accumulation vs the same rounding on the CPU), one Adam step; then training,
 per-macrocell validation and rendering with the network in the element stage.
 
-Tolerances: the GPU and the oracle round the same values to bf16 but sum in
-different orders, so a few bf16 roundings flip by one unit (2^-8 relative);
-whole-array relative errors stay below 2e-2 and correlations above 0.999.
-Training and rendering are judged against the exact sampler: the trained
-network's image must stay within the noise of two exact images.
+Tolerances: the GPU and the oracle round the same values to bf16 and sum in
+float32 in different orders; gradients agree to about 1e-7 whole-array
+relative error, so they are held to 1e-4 (a 1% scale error in one layer
+fails), forward values to 2e-3 (a bf16 rounding flips now and then).
+Training and rendering are judged against the exact sampler at the
+configuration's validation bound, 1e-3 of the value range: validated
+macrocells' majorants are the TF maxima over the exact value range widened
+by the bound, and the image with the network stays within the noise of two
+exact images.
 """
 import numpy as np
 import pytest
@@ -80,10 +84,10 @@ def test_gradients_match_oracle(renderer):
     loss_o, (dt_o, d1_o, d2_o, d3_o) = onn.loss_and_grads(cfg, params, x, np.where(valid, t, 0), valid)
     print(f"loss {loss:.6g} vs {loss_o:.6g}; rel dW1 {rel(d1, d1_o):.3g} dW2 {rel(d2, d2_o):.3g} "
           f"dw3 {rel(d3, d3_o):.3g} dtable {rel(dt, dt_o):.3g}")
-    assert abs(loss - loss_o) <= 1e-3 * loss_o
+    assert abs(loss - loss_o) <= 1e-4 * loss_o
     for g, go in ((d1, d1_o), (d2, d2_o), (d3, d3_o), (dt, dt_o)):
-        assert rel(g, go) < 2e-2
-        assert np.corrcoef(g.ravel(), go.ravel())[0, 1] > 0.999
+        assert rel(g, go) < 1e-4
+        assert np.corrcoef(g.ravel(), go.ravel())[0, 1] > 0.999999
     # rows without a target contribute nothing: the same batch without them
     keep = np.flatnonzero(valid)[:192]
     loss2, g2 = f.gradients(x[keep], t[keep])
@@ -109,8 +113,8 @@ def test_forward_matches_oracle(renderer, device):
     y, _ = onn.forward(cfg, params, x01)
     want = vr[0] + y * (vr[1] - vr[0])
     print(f"forward rel err {rel(got, want):.3g}")
-    assert rel(got - vr[0], want - vr[0]) < 2e-2
-    assert np.corrcoef(got, want)[0, 1] > 0.999
+    assert rel(got - vr[0], want - vr[0]) < 2e-3
+    assert np.corrcoef(got, want)[0, 1] > 0.99999
 
 
 def test_adam_step_matches_oracle(renderer):
@@ -135,7 +139,7 @@ def trained(renderer):
     f = NeuralField(renderer, levels=16, log2T=16, n_min=16, n_max=128, seed=1)
     nvalid = f.makePool(1 << 20, 0.7, seed=7)
     assert nvalid > 0.9 * (1 << 20)
-    loss = f.train(1500, batch=1 << 14, lr=1e-2)
+    loss = f.train(3000, batch=1 << 14, lr=1e-2, lr_end=1e-4)
     return f, loss
 
 
@@ -151,13 +155,31 @@ def test_validate_and_render_with_network(trained, renderer, device):
     of the accepted macrocells; the image stays within the seed noise of the
     exact sampler's."""
     import torch
+    from oracle import oracle as ora
     f, _ = trained
     vr = renderer.valueRange
-    v = f.validate(points_per_mc=64, bound=2e-2 * float(vr[1] - vr[0]))
+    v = f.validate(points_per_mc=64, bound=1e-3 * float(vr[1] - vr[0]))
     print(f"validation: {v}", flush=True)
     assert v["neural_mcs"] > 0 and v["max_error"] <= v["bound"]
     flags = f.flags(32 ** 3)
     assert flags.sum() == v["neural_mcs"]
+    # the majorants the renderer uses with the network: the TF maximum over
+    # the exact value range widened by the bound in validated macrocells
+    # (network values are clamped into that range), the exact one elsewhere
+    st = renderer.state()
+    rgba = scenes.tf128()
+    xr = np.array(st.tfDomain, np.float32)
+    ranges = renderer.sampler.value_ranges()
+    wid = ranges.copy()
+    sel = (flags == 1) & (ranges[:, 0] <= ranges[:, 1])
+    b32 = np.float32(v["bound"])
+    wid[sel, 0] = ranges[sel, 0] - b32
+    wid[sel, 1] = ranges[sel, 1] + b32
+    renderer.setNeural(f)
+    got = renderer.sampler.max_opacities()
+    renderer.setNeural(None)
+    assert np.array_equal(got, ora.map_max_opacity(wid, rgba, xr))
+    assert np.array_equal(renderer.sampler.max_opacities(), ora.map_max_opacity(ranges, rgba, xr))
     W, H = renderer.size
     fb = torch.zeros((H, W), dtype=torch.int32, device=device)
     K = 8
@@ -197,7 +219,7 @@ def test_batch_with_network_matches_single_view(trained, renderer, device):
     from owlexastitcher_amd import RenderBatch
     f, _ = trained
     vr = renderer.valueRange
-    f.validate(points_per_mc=64, bound=2e-2 * float(vr[1] - vr[0]))
+    f.validate(points_per_mc=64, bound=1e-3 * float(vr[1] - vr[0]))
     W, H = renderer.size
     st = renderer.state()
     cam = (np.array(st.camOrg), np.array(st.camDir00), np.array(st.camDu), np.array(st.camDv))
