@@ -22,6 +22,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "path-traced samples/sec @1080p on 1 GPU; neural vs exact sampler at fixed RMSE"
 HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0  # dense bf16 MFMA
 
 # algorithmic bytes per unit of work (leaf records only, traversal nodes are
 # overhead): gridlet hit = 32 B record + 8 corners x 4 B; element hit = 32 B
@@ -410,6 +411,27 @@ def neural_workload(r, args, dev):
     rmse = float(np.sqrt(np.mean((N - A)[..., :3] ** 2)))
     noise = float(np.sqrt(np.mean((B - A)[..., :3] ** 2)))
     samples = args.width * args.height * args.spp
+    # MFMA rate of the network kernels: training (forward + backward of a
+    # batch, 3 x 12 416 FLOP per sample, the count rocprofv3's
+    # SQ_INSTS_VALU_MFMA_MOPS_BF16 x 512 reproduces) and inference over 2^22
+    # points (12 416 FLOP each), against the dense bf16 peak
+    flop_sample = 2 * (32 * 64 + 64 * 64 + 64 * 1)
+    train_tflops = 3 * flop_sample * args.train_batch / (train_s / args.train_steps) / 1e12
+    b = r.modelBounds
+    n_eval = 1 << 22
+    d_pos = torch.from_numpy(np.random.default_rng(args.seed).uniform(b[:3], b[3:], (n_eval, 3)).astype(np.float32)).to(dev)
+    d_val = torch.empty(n_eval, dtype=torch.float32, device=dev)
+    f.evaluate(d_pos, d_val, stream)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(5):
+        f.evaluate(d_pos, d_val, stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    eval_ms = e0.elapsed_time(e1) / 5
+    eval_tflops = flop_sample * n_eval / (eval_ms * 1e-3) / 1e12
+    log(f"[bench] network MFMA rate: training {train_tflops:.2f} TFLOP/s ({100 * train_tflops / BF16_PEAK_TFLOPS:.2f}% "
+        f"of the bf16 peak), inference {eval_tflops:.1f} TFLOP/s ({100 * eval_tflops / BF16_PEAK_TFLOPS:.2f}%)")
     log(f"[bench] exact {ms_exact:.2f} ms/frame, neural {ms_net:.2f} ms/frame; image RMSE {rmse:.3g} "
         f"(seed noise {noise:.3g}); element lookups exact {sum(c_exact[k] for k in ('tet', 'pyr', 'wedge', 'hex'))}, "
         f"with network: {c_net['neural']} by the network + "
@@ -426,8 +448,12 @@ def neural_workload(r, args, dev):
                    "train_batch": args.train_batch, "train_seconds": train_s, "pool": args.pool,
                    "lr": [1e-2, args.lr_end],
                    "final_loss": float(np.mean(loss[-50:])), "counters_neural": c_net,
-                   "counters_exact": c_exact},
-        "roofline": None, "cpu_baseline": None,
+                   "counters_exact": c_exact, "train_step_ms": train_s / args.train_steps * 1e3,
+                   "inference_ms_per_2e22_points": eval_ms, "inference_tflops": eval_tflops,
+                   "inference_mfma_frac": eval_tflops / BF16_PEAK_TFLOPS},
+        "roofline": {"bound": "mfma", "achieved": train_tflops, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": train_tflops / BF16_PEAK_TFLOPS, "traffic": None, "kernel": "kNeuralTrain"},
+        "cpu_baseline": None,
     }
 
 
