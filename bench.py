@@ -738,9 +738,9 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--accel", choices=["tables", "bvh"], default="tables")
     ap.add_argument("--scheme", choices=["persistent", "megakernel"], default="persistent")
-    ap.add_argument("--ctrl-steps", type=int, default=6)
+    ap.add_argument("--ctrl-steps", type=int, default=4)
     ap.add_argument("--elem-batch", type=int, default=16)
-    ap.add_argument("--rare-batch", type=int, default=16)
+    ap.add_argument("--rare-batch", type=int, default=24)
     ap.add_argument("--element-stage", type=int, default=0, choices=[0, 1, 2],
                     help="0: in the waiting lanes' wave; 1: workgroup queue in LDS; 2: device queue between launches")
     ap.add_argument("--queue-rounds", type=int, default=1, help="element stage 2: queue rounds per frame")

@@ -276,7 +276,7 @@ int exa_renderer_set_accel(exa_renderer *r, int accel);
    step into the next macrocell when it leaves the cell) per lane and pass
    (>= 1); elemBatch and rareBatch in [0, 64] = wave fraction (x/64) of live
    lanes that must wait for an element test, resp. for per-segment work,
-   before that pass runs.  Defaults 6, 16, 16. */
+   before that pass runs.  Defaults 4, 16, 24. */
 #define EXA_RENDER_MEGAKERNEL 0
 #define EXA_RENDER_PERSISTENT 1
 int exa_renderer_set_scheme(exa_renderer *r, int scheme);

@@ -171,9 +171,9 @@ constexpr int kMaxBatchRequests = 1024;
 
 // scheduling knobs of the persistent renderer (results do not depend on them)
 struct PersistTuning {
-  int32_t ctrlSteps = 6;   // tracking iterations per lane and round, >= 1
+  int32_t ctrlSteps = 4;   // tracking iterations per lane and round, >= 1
   int32_t elemBatch = 16;  // element pass when waiting lanes >= elemBatch/64 of live lanes
-  int32_t rareBatch = 16;  // per-segment pass when waiting lanes >= rareBatch/64 of live lanes
+  int32_t rareBatch = 24;  // per-segment pass when waiting lanes >= rareBatch/64 of live lanes
   // where the element stage runs: kElemInWave (the waiting lanes' own wave,
   // gated by elemBatch), kElemWorkgroupQueue (any wave of the workgroup
   // answers up to 64 queries posted in LDS, once elemBatch are posted),

@@ -561,7 +561,7 @@ class OWLRenderer:
             check(rc)
         return rc
 
-    def setPersistentTuning(self, ctrlSteps=6, elemBatch=16, rareBatch=16):
+    def setPersistentTuning(self, ctrlSteps=4, elemBatch=16, rareBatch=24):
         """Scheduling knobs of RENDER_PERSISTENT; images do not depend on them."""
         check(lib().exa_renderer_set_persistent_tuning(self.h, int(ctrlSteps), int(elemBatch), int(rareBatch)))
 

@@ -48,7 +48,7 @@ def workload(amr):
     r.setMaxBounces(4)
     r.setSpp(1)
     r.setDirectionalLight(0, (0.3, 1.0, 0.5), 2.0, True)
-    r.setPersistentTuning(6, 16, 16)
+    r.setPersistentTuning(4, 16, 24)
     idx, verts, gl, gs = dm.arrays()
     assert len(idx) > 5_000_000 and len(gl) > 100_000
     st = r.state()
