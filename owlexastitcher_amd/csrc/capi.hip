@@ -165,7 +165,8 @@ struct exa_sampler {
   float bandMargin = 0.00390625f, fastMargin = 0.0078125f;  // gridletMargins of the model
   DeviceBuffer<uint32_t> tGStart, tEStart;
   DeviceBuffer<int32_t> tGList, tGTop, tGOwner;
-  DeviceBuffer<float4> tECand, tEVerts;
+  DeviceBuffer<uint3> tECand;
+  DeviceBuffer<float4> tEVerts;
   // gridlet value ranges + per-primitive max opacities (culling)
   DeviceBuffer<float2> gridletRanges;
   DeviceBuffer<float> gridletMaxOp, elemMaxOp;
@@ -241,7 +242,7 @@ struct exa_sampler {
     tGTop.upload(t.gTop.data(), t.gTop.size(), s);
     tGOwner.upload(t.gOwner.data(), t.gOwner.size(), s);
     tEStart.upload(t.eStart.data(), t.eStart.size(), s);
-    tECand.upload((const float4 *)t.eCand.data(), t.eCand.size() / 4, s);
+    tECand.upload((const uint3 *)t.eCand.data(), t.eCand.size() / 3, s);
     tEVerts.upload((const float4 *)t.eVerts.data(), t.eVerts.size() / 4, s);
     HIP_CHECK(hipStreamSynchronize(s));
     tabMeta = LocateTables();

@@ -172,8 +172,11 @@ struct LocateTables {
   float binInv = 1.f;  // 1 / bin size (a power of two)
   int32_t binDims[3] = {0, 0, 0};
   std::vector<uint32_t> eStart;  // per bin + 1
-  // one record per list entry: lo.xyz + element id (int bits), hi.xyz + vertex count
-  std::vector<float> eCand;
+  // three words per list entry: element id | vertex count << 28, then the
+  // element's box quantised to 512 sub-cells of the entry's bin (9 bits per
+  // bound in 10-bit slots at bits 0, 10, 20: lo with the slots' top bits
+  // clear, hi with them set)
+  std::vector<uint32_t> eCand;
   // per element its vertices (xyz + scalar) in index order, 8 slots
   std::vector<float> eVerts;
 };

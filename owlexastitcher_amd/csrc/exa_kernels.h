@@ -56,7 +56,7 @@ struct SamplerView {
   const int32_t *gTop;    // nullptr: owner tables not built
   const int32_t *gOwner;
   const uint32_t *eStart;
-  const float4 *eCand;   // 2 per list entry: lo.xyz + id, hi.xyz + vertex count
+  const uint3 *eCand;    // per list entry: id | vertex count << 28, quantised box lo, hi
   const float4 *eVerts;  // 8 per element
   float binLo[3];
   float binInv;

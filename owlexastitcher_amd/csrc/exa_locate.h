@@ -96,25 +96,6 @@ EXA_DEV int testElement(float &value, vec3f P, const int32_t *__restrict__ idx8,
   return nv == 4 ? 0 : (nv == 5 ? 1 : (nv == 6 ? 2 : 3));
 }
 
-// Element test on an element's own vertex record (8 float4 slots holding the
-// same values as vertices[indices[...]]); same arithmetic as testElement.
-EXA_DEV int testElementRecord(float &value, vec3f P, int nv, const float4 *__restrict__ rec) {
-  float4 v[8];
-  v[0] = rec[0];
-  v[1] = rec[1];
-  v[2] = rec[2];
-  v[3] = rec[3];
-  if (nv > 4) v[4] = rec[4];
-  if (nv > 5) v[5] = rec[5];
-  if (nv > 6) { v[6] = rec[6]; v[7] = rec[7]; }
-  bool hit;
-  if (nv == 4) hit = intersectTet(value, P, v[0], v[1], v[2], v[3]);
-  else if (nv == 5 || nv == 6 || nv == 8) hit = intersectNewtonAny(value, P, v, nv);
-  else hit = false;
-  if (!hit) return -1;
-  return nv == 4 ? 0 : (nv == 5 ? 1 : (nv == 6 ? 2 : 3));
-}
-
 // Direct-table point location (LocateTables), gridlet stage: per level, the
 // gridlet list of macrocell floor((p/2^L - .5)/8).  A hit strictly inside a
 // gridlet cell ends the search (dual cells of one mesh tile space, so no
@@ -258,48 +239,99 @@ EXA_DEV Sample lookupTablesGridlets(const SamplerView &sv, const Culling &cu, co
   return locateTablesGridlets(sv, cu, p);
 }
 
-// Direct-table point location, element stage: the element list of p's bin,
-// stored AABB first; kind receives 1..4 (tet/pyr/wedge/hex) or -1.
+// Element test on an element's own vertex record (8 float4 slots holding the
+// same values as vertices[indices[...]]), behind the exact AABB check when
+// the caller cannot rule it out (boxCheck): the box of its nv vertices (the
+// float box the bins were built from: the same min/max over the same
+// floats).  Then the arithmetic of testElement.  -1 or 0..3
+// (tet/pyr/wedge/hex).
+EXA_DEV int testElementBoxed(float &value, vec3f P, int nv, const float4 *__restrict__ rec, bool boxCheck) {
+  if (nv < 4) return -1;
+  float4 v[8];
+  v[0] = rec[0];
+  v[1] = rec[1];
+  v[2] = rec[2];
+  v[3] = rec[3];
+  if (nv > 4) v[4] = rec[4];
+  if (nv > 5) v[5] = rec[5];
+  if (nv > 6) { v[6] = rec[6]; v[7] = rec[7]; }
+  if (boxCheck) {
+    float lx = v[0].x, ly = v[0].y, lz = v[0].z, hx = lx, hy = ly, hz = lz;
+#pragma unroll
+    for (int i = 1; i < 8; ++i)
+      if (i < nv) {
+        lx = fminf(lx, v[i].x);
+        ly = fminf(ly, v[i].y);
+        lz = fminf(lz, v[i].z);
+        hx = fmaxf(hx, v[i].x);
+        hy = fmaxf(hy, v[i].y);
+        hz = fmaxf(hz, v[i].z);
+      }
+    if (!(P.x >= lx && P.y >= ly && P.z >= lz && P.x <= hx && P.y <= hy && P.z <= hz)) return -1;
+  }
+  bool hit;
+  if (nv == 4) hit = intersectTet(value, P, v[0], v[1], v[2], v[3]);
+  else if (nv == 5 || nv == 6 || nv == 8) hit = intersectNewtonAny(value, P, v, nv);
+  else hit = false;
+  if (!hit) return -1;
+  return nv == 4 ? 0 : (nv == 5 ? 1 : (nv == 6 ? 2 : 3));
+}
+
+// Direct-table point location, element stage: the element list of p's bin;
+// kind receives 1..4 (tet/pyr/wedge/hex) or -1.  Each 12-byte entry holds the
+// element's box quantised to 512 sub-cells of the bin, 9 bits per bound in
+// 10-bit slots, lo with the slots' top bits clear and hi with them set.  With
+// p's sub-cells in the same slots, one subtraction compares three bounds:
+// slot by slot it computes 512 + a - b (1..1023, so no borrow crosses a slot)
+// and the slot's top bit survives exactly when a >= b.  An entry whose box
+// misses p's sub-cell is rejected without touching the element.  If p's
+// sub-cell lies strictly inside the quantised box, p lies inside the float
+// box; otherwise the exact box decides.  The quantised box is conservative,
+// so the first accepting element in ID order is the one the float boxes give.
 EXA_DEV Sample locateTablesElements(const SamplerView &sv, const Culling &cu, const vec3f p, int &kind) {
+  constexpr uint32_t kTop = 0x20080200u;  // bit 9 of each 10-bit slot
+  constexpr uint32_t kOne = 0x00100401u;  // bit 0 of each slot
   Sample s{-1, -1, 0.f};
   kind = -1;
   if (sv.numElements <= 0) return s;
   int32_t b[3];
+  uint32_t pq = 0;
   const float q[3] = {p.x, p.y, p.z};
+  const float inv512 = sv.binInv * 512.f;
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     const float f = floorf((q[k] - sv.binLo[k]) * sv.binInv);
     b[k] = f != f ? 0 : (f < 0.f ? 0 : (f > (float)(sv.binDims[k] - 1) ? sv.binDims[k] - 1 : (int32_t)f));
+    const float g = floorf((q[k] - sv.binLo[k]) * inv512) - (float)b[k] * 512.f;
+    pq |= (g != g ? 0u : (g < 0.f ? 0u : (g > 511.f ? 511u : (uint32_t)g))) << (10 * k);
   }
+  const uint32_t pTop = pq | kTop;
   const int64_t bin = ((int64_t)b[2] * sv.binDims[1] + b[1]) * sv.binDims[0] + b[0];
   const uint32_t k0 = sv.eStart[bin], k1 = sv.eStart[bin + 1];
-  // candidates in ID order, four records per batch of independent loads;
-  // the AABB tests of a batch give a mask, then one element test per set bit
-  // (lanes test their own candidates together, whatever the element types)
+  // entries in ID order, four per batch of independent loads; the sub-cell
+  // tests of a batch give a mask, then one exact test per set bit (lanes
+  // test their own candidates together, whatever the element types)
   for (uint32_t k = k0; k < k1; k += 4) {
-    float4 lo[4], hi[4];
+    uint3 c[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j)
-      if (k + j < k1) {
-        lo[j] = sv.eCand[2 * (size_t)(k + j)];
-        hi[j] = sv.eCand[2 * (size_t)(k + j) + 1];
-      }
-    unsigned hits = 0;
+      if (k + j < k1) c[j] = sv.eCand[k + j];
+    unsigned hits = 0, inner = 0;
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
-      if (k + j < k1 && p.x >= lo[j].x && p.y >= lo[j].y && p.z >= lo[j].z && p.x <= hi[j].x &&
-          p.y <= hi[j].y && p.z <= hi[j].z)
-        hits |= 1u << j;
+    for (int j = 0; j < 4; ++j) {
+      const uint32_t dLo = pTop - c[j].y, dHi = c[j].z - pq;
+      if (k + j < k1 && (dLo & dHi & kTop) == kTop) hits |= 1u << j;
+      if (((dLo - kOne) & (dHi - kOne) & kTop) == kTop) inner |= 1u << j;
+    }
     while (hits) {
       const int j = __builtin_ctz(hits);
       hits &= hits - 1;
-      // the candidate's id and vertex count (selects, not an indexed array)
-      const float ew = j == 0 ? lo[0].w : (j == 1 ? lo[1].w : (j == 2 ? lo[2].w : lo[3].w));
-      const float nw = j == 0 ? hi[0].w : (j == 1 ? hi[1].w : (j == 2 ? hi[2].w : hi[3].w));
-      const int32_t e = __float_as_int(ew);
+      // the candidate's head word (selects, not an indexed array)
+      const uint32_t h = j == 0 ? c[0].x : (j == 1 ? c[1].x : (j == 2 ? c[2].x : c[3].x));
+      const int32_t e = (int32_t)(h & 0x0fffffffu);
       if (cu.elemMaxOp && cu.elemMaxOp[e] == 0.f) continue;
       float v;
-      const int t = testElementRecord(v, p, __float_as_int(nw), sv.eVerts + 8 * (int64_t)e);
+      const int t = testElementBoxed(v, p, (int)(h >> 28), sv.eVerts + 8 * (int64_t)e, !((inner >> j) & 1u));
       if (t >= 0) {
         s.primID = e;
         s.cellID = -1;

@@ -11,14 +11,25 @@
 
 namespace exa {
 
-// bin index of coordinate x: the same float expression runs on the device
-// device-memory budget of the element candidate records
-static constexpr int64_t kMaxCandidateBytes = 8ll << 30;
+// budget of element-bin list entries (the bins are coarsened past it)
+static constexpr int64_t kMaxCandidates = 1ll << 28;
 
+// bin index of coordinate x: the same float expression runs on the device
 static inline int32_t binOf(float x, float lo, float inv, int32_t dims) {
   const float f = floorf((x - lo) * inv);
   int32_t i = f != f ? 0 : (f < 0.f ? 0 : (f > (float)(dims - 1) ? dims - 1 : (int32_t)f));
   return i;
+}
+
+// Sub-cell 0..511 of coordinate x inside bin b of its axis (512 sub-cells
+// per bin, clamped).  The same float expression runs on the device.  It is
+// monotone in x, so lo <= x <= hi gives subCellOf(lo) <= subCellOf(x) <=
+// subCellOf(hi) in every bin: a box quantised this way never rejects a point
+// the float box holds, and a point whose sub-cell lies strictly between the
+// quantised bounds lies strictly inside the float box.
+static inline uint32_t subCellOf(float x, float lo, float inv512, int32_t b) {
+  const float g = floorf((x - lo) * inv512) - (float)b * 512.f;
+  return g != g ? 0u : (g < 0.f ? 0u : (g > 511.f ? 511u : (uint32_t)g));
 }
 
 // Dual-cell owner tables.  Gridlet IDs ascend with level, so per level the
@@ -141,8 +152,8 @@ void buildLocateTables(const HostModel &m, LocateTables &t) {
   }
   std::nth_element(ext.begin(), ext.begin() + ne / 2, ext.end());
   // Bin size: the power of two at or below half the median element extent
-  // (a few AABB candidates per query), doubled while the bins or their
-  // 32-byte candidate records would outgrow their budget.
+  // (a few AABB candidates per query), doubled while the bins or their list
+  // entries would outgrow their budget.
   float size = 1.f;
   while (size * 2.f <= ext[ne / 2] && size < 1024.f) size *= 2.f;
   size *= 0.5f;
@@ -160,7 +171,7 @@ void buildLocateTables(const HostModel &m, LocateTables &t) {
         n *= binOf(boxes[6 * e + 3 + k], lo[k], 1.f / size, t.binDims[k]) - binOf(boxes[6 * e + k], lo[k], 1.f / size, t.binDims[k]) + 1;
       entries += n;
     }
-    if (entries * 32 > kMaxCandidateBytes && size < 1024.f) {
+    if (entries > kMaxCandidates && size < 1024.f) {
       size *= 2.f;
       continue;
     }
@@ -197,28 +208,36 @@ void buildLocateTables(const HostModel &m, LocateTables &t) {
   t.eStart.assign(nb + 1, 0);
   for (int64_t i = 0; i < nb; ++i) t.eStart[i + 1] = t.eStart[i] + bc[i].load(std::memory_order_relaxed);
   if (t.eStart[nb] >= (1u << 31)) throw std::runtime_error("locate tables: element bin lists too large");
-  t.eCand.resize(8 * (size_t)t.eStart[nb]);
+  if (ne >= (1ll << 28)) throw std::runtime_error("locate tables: over 2^28 stitching elements");
+  // List entries, 3 words: id | vertex count << 28, then the box quantised
+  // to the entry's bin, 9 bits per bound in 10-bit slots (x, y, z at bits 0,
+  // 10, 20): the lo word with every slot's top bit clear, the hi word with
+  // every slot's top bit set.
+  t.eCand.resize(3 * (size_t)t.eStart[nb]);
   std::vector<uint32_t> cur(t.eStart.begin(), t.eStart.end() - 1);
+  const float inv512 = t.binInv * 512.f;
   for (int64_t e = 0; e < ne; ++e) {
     int32_t a[3], b[3];
     range(e, a, b);
     const int32_t *row = &m.indices[8 * e];
-    int32_t nv = 0;
+    uint32_t nv = 0;
     while (nv < 8 && row[nv] >= 0) nv++;
-    float rec[8];
-    const int32_t eid = (int32_t)e;
-    rec[0] = boxes[6 * e];
-    rec[1] = boxes[6 * e + 1];
-    rec[2] = boxes[6 * e + 2];
-    memcpy(&rec[3], &eid, 4);
-    rec[4] = boxes[6 * e + 3];
-    rec[5] = boxes[6 * e + 4];
-    rec[6] = boxes[6 * e + 5];
-    memcpy(&rec[7], &nv, 4);
-    for (int z = a[2]; z <= b[2]; ++z)
-      for (int y = a[1]; y <= b[1]; ++y)
-        for (int x = a[0]; x <= b[0]; ++x)
-          memcpy(&t.eCand[8 * (size_t)cur[((int64_t)z * t.binDims[1] + y) * t.binDims[0] + x]++], rec, 32);
+    const uint32_t head = (uint32_t)e | nv << 28;
+    const float *bx = &boxes[6 * e];
+    for (int z = a[2]; z <= b[2]; ++z) {
+      const uint32_t lz = subCellOf(bx[2], t.binLo[2], inv512, z) << 20;
+      const uint32_t hz = subCellOf(bx[5], t.binLo[2], inv512, z) << 20 | 0x20080200u;
+      for (int y = a[1]; y <= b[1]; ++y) {
+        const uint32_t ly = lz | subCellOf(bx[1], t.binLo[1], inv512, y) << 10;
+        const uint32_t hy = hz | subCellOf(bx[4], t.binLo[1], inv512, y) << 10;
+        for (int x = a[0]; x <= b[0]; ++x) {
+          const size_t k = cur[((int64_t)z * t.binDims[1] + y) * t.binDims[0] + x]++;
+          t.eCand[3 * k] = head;
+          t.eCand[3 * k + 1] = ly | subCellOf(bx[0], t.binLo[0], inv512, x);
+          t.eCand[3 * k + 2] = hy | subCellOf(bx[3], t.binLo[0], inv512, x);
+        }
+      }
+    }
   }
   t.eVerts.assign(32 * (size_t)ne, 0.f);
   for (int64_t e = 0; e < ne; ++e)

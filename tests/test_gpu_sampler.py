@@ -237,3 +237,44 @@ def test_elements_only_model(cfg1, device, accel):
     om.set_culling(None, om.element_max_opacity(rgba, np.array(st.tfDomain, np.float32)))
     acc_o, _ = ora.render_f32(ora.render_params_from_state(st, maj, rgba), om)
     assert np.array_equal(accs[0], acc_o) and np.array_equal(accs[1], acc_o)
+
+
+@pytest.mark.parametrize("accel", [ACCEL_TABLES, ACCEL_BVH], ids=["tables", "bvh"])
+def test_locate_on_element_box_faces(noise_scene, device, accel):
+    """Points on and one float step either side of stitching-element box
+    faces, edges and corners (the element-bin lists store each box quantised
+    to sub-cells of the bin, so these are the points where only the exact box
+    decides) against the oracle, and against brute force on a subset."""
+    print(f"case: locate on element box faces, noise scene, accel {accel}", flush=True)
+    sc = noise_scene
+    sc.model.setNumGridCells((16, 16, 16))
+    s = ExaStitchSampler()
+    s.setAccel(accel)
+    s.build(sc.model)
+    rng = np.random.default_rng(31)
+    ne = len(sc.idx)
+    e = rng.integers(0, ne, size=40000)
+    v = sc.verts[:, :3]
+    nv = np.count_nonzero(sc.idx >= 0, axis=1)
+    corners = np.where((np.arange(8)[None, :] < nv[e, None])[..., None], v[np.maximum(sc.idx[e], 0)], np.nan)
+    lo, hi = np.nanmin(corners, axis=1), np.nanmax(corners, axis=1)
+    pos = rng.uniform(lo, hi).astype(np.float32)
+    # snap one to three axes onto a face of the box, then nudge some of
+    # those one float step inward or outward
+    snap = rng.random((len(e), 3)) < 0.5
+    side = rng.random((len(e), 3)) < 0.5
+    face = np.where(side, hi, lo).astype(np.float32)
+    pos = np.where(snap, face, pos)
+    step = rng.integers(-1, 2, size=(len(e), 3))
+    out = np.nextafter(pos, np.where(step > 0, np.inf, -np.inf).astype(np.float32))
+    pos = np.where(snap & (step != 0), out, pos).astype(np.float32)
+    ids, vals = _locate_gpu(s, pos, device)
+    om = ora.Model(sc.verts, sc.idx, sc.gridlets, sc.gscalars)
+    ref = om.locate_binned(pos)
+    mism, rel, _ = _compare(ids, vals, ref)
+    el_hits = int(np.count_nonzero((ref[:, 0] >= 0) & (ref[:, 1] < 0)))
+    print(f"id mismatches {mism}, max rel {rel:.3g}, element hits {el_hits} of {len(pos)}")
+    assert mism == 0 and rel <= VALUE_RTOL
+    assert el_hits > len(pos) // 4
+    ref_b, _ = om.locate_brute(pos[:3000])
+    assert np.array_equal(ids[:3000], ref_b[:, :2])
