@@ -65,10 +65,13 @@ int exa_amr_create_noise(int baseRes, int baseLevel, float thresh1, float thresh
                          float contrast, uint64_t seed, exa_amr **out);
 /* Expanding spherical shock shell with turbulence in its wake (the large
    configuration: baseRes 256, baseLevel 4, radius 1200, thickness 40 over
-   [0,4096]^3): shell at level 1, level-0 patches where the turbulence exceeds
-   thetaCore, level >= 2 in the wake where it exceeds thetaWake; leaf levels
-   follow a graded distance field, so touching leaves differ by at most one
-   level.  Scalar = shock profile x turbulence + wake turbulence, in [0,1]. */
+   [0,4096]^3): shell at level 1, level-0 patches where a low-frequency
+   noise exceeds thetaCore, inside the shell levels 2 and 3 split along the
+   iso-surface thetaWake of a cell-scale turbulence (level 3 below it; about
+   16% stitching elements per leaf cell at the default size with thetaWake
+   0.43); leaf levels follow a graded distance field, so touching leaves
+   differ by at most one level.  Scalar = shock profile x turbulence + wake
+   turbulence, in [0,1]. */
 int exa_amr_create_shock(int baseRes, int baseLevel, float radius, float thickness, float thetaCore,
                          float thetaWake, uint64_t seed, exa_amr **out);
 /* leaf cells per level (counts[16]) of that scene, without building it */

@@ -239,24 +239,31 @@ void amrNoise(AmrScene &out, int baseRes, int baseLevel, float thresh1, float th
 // The leaf level of a point follows a graded distance field
 //   T(x) = floor(log2(max(1, D(x) / G))),  clamped to [0, baseLevel],
 // D = min(D_shell, D_core, D_wake), with ds the distance to a noise-perturbed
-// sphere minus the shell half thickness and n a low-frequency turbulence:
+// sphere minus the shell half thickness, n a low-frequency noise and w a
+// cell-scale turbulence:
 //   D_shell = 2.5G + k ds                          (shell band at level 1),
 //   D_core  = G + k ds + S max(0, thetaCore - n)   (level-0 patches in it),
-//   D_wake  = 4.75G + S max(0, thetaWake - n) + max(0, r - R)
-//                                                  (level-2 wake inside);
-// the offsets above 2G and 4G leave room for the split test's margin.
+//   D_wake  = max(4.75G, Dw + Sw (thetaWake - w)) + max(0, r - R)
+//                                 (levels 2 and 3 inside, split along w);
+// the offsets above 2G and 4G leave room for the split test's margin, and Dw
+// is the level-3 split threshold, so the wake's level-2/3 boundary follows
+// the iso-surface w = thetaWake (features of about 10 units, which gives the
+// dual mesh about 16% stitching elements per leaf cell at the default size
+// with thetaWake 0.43).
 // A cell of level L is split when a lower bound of D over the cell (D at the
 // centre minus Lip * half diagonal) is below G 2^L.  For a field with
 // Lipschitz constant Lip and G > 6.1 Lip, touching leaves differ by at most
 // one level (vertex 2:1 balance, which the dual mesh needs).  With value
 // noise of slope at most 2.6 f per octave, k = 1.5, S = 400 (per 4096 of
-// domain) and the noise frequencies below bound Lip by 2.5 (G = 16).  The scalar field uses a finer
-// turbulence than the refinement.
+// domain), the frequencies of n below, w of 2 octaves from 1.5/G per unit and
+// Sw = 4.3 (slope at most 4.3 * 2.6 * 0.09375 * 2/1.5 = 1.40, plus 1 from
+// r - R) bound Lip by 2.5 (G = 16).  The scalar field uses a finer turbulence
+// than the refinement.
 // ---------------------------------------------------------------------------
 namespace {
 struct ShockField {
-  double cx, cy, cz, R, h, G, lip, k, slope, wobble;
-  Fbm shellNoise, refine, turb;
+  double cx, cy, cz, R, h, G, lip, k, slope, wobble, wakeSlope, wakeOffset;
+  Fbm shellNoise, refine, turb, wakeTurb;
   double thetaCore, thetaWake;
   double dShell(double x, double y, double z, double &r) const {
     const double dx = x - cx, dy = y - cy, dz = z - cz;
@@ -270,7 +277,8 @@ struct ShockField {
     const double n = refine(x, y, z);
     const double shell = 2.5 * G + k * ds;
     const double core = G + k * ds + slope * std::max(0.0, thetaCore - n);
-    const double wake = 4.75 * G + slope * std::max(0.0, thetaWake - n) + std::max(0.0, r - R);
+    const double wake =
+        std::max(4.75 * G, wakeOffset + wakeSlope * (thetaWake - wakeTurb(x, y, z))) + std::max(0.0, r - R);
     return std::min(std::min(shell, core), wake);
   }
   double scalar(double x, double y, double z) const {
@@ -298,6 +306,9 @@ ShockField makeShock(double domain, float radius, float thickness, float thetaCo
   F.shellNoise = Fbm{2, 3.0 / domain, 1.0, seed};
   F.refine = Fbm{2, 2.0 / domain, 1.0, seed + 5};
   F.turb = Fbm{3, 6.0 / domain, 1.0, seed + 7};
+  F.wakeTurb = Fbm{2, 1.5 / F.G, 1.0, seed + 11};
+  F.wakeSlope = 4.3;
+  F.wakeOffset = 8.0 * F.G + F.lip * 0.8660254037844386 * 8.0;
   F.thetaCore = thetaCore;
   F.thetaWake = thetaWake;
   return F;

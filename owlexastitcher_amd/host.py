@@ -68,7 +68,7 @@ class AmrScene:
         return cls(h)
 
     @classmethod
-    def shock(cls, base_res=256, base_level=4, radius=1200.0, thickness=40.0, theta_core=0.74, theta_wake=0.55,
+    def shock(cls, base_res=256, base_level=4, radius=1200.0, thickness=40.0, theta_core=0.74, theta_wake=0.43,
               seed=1):
         """Expanding spherical shock shell with turbulence in its wake (5 levels
         at the defaults, over [0, base_res * 2^base_level]^3)."""
@@ -78,7 +78,7 @@ class AmrScene:
         return cls(h)
 
     @staticmethod
-    def shock_counts(base_res=256, base_level=4, radius=1200.0, thickness=40.0, theta_core=0.74, theta_wake=0.55,
+    def shock_counts(base_res=256, base_level=4, radius=1200.0, thickness=40.0, theta_core=0.74, theta_wake=0.43,
                      seed=1):
         """Leaf cells per level of AmrScene.shock(...) without building it."""
         c = np.zeros(16, np.int64)

@@ -1,8 +1,8 @@
 """The large configuration at its full size against the oracle.
 
-5-level shock-shell AMR (base 256^3 of width 16 over [0,4096]^3, 407.9M leaf
-cells), its dual mesh (21.1M stitching elements, 0.95M gridlets with 611M
-corner scalars), 64^3 majorant grid, the 3840x2160 main view of the large
+5-level shock-shell AMR (base 256^3 of width 16 over [0,4096]^3, 383M leaf
+cells), its dual mesh (about 63M stitching elements, 16% of the leaf cells;
+0.97M gridlets with 612M corner scalars), 64^3 majorant grid, the 3840x2160 main view of the large
 workload at 1 spp: the oracle's majorant grid and per-primitive culling
 against the library's, then every eighth image row (270 rows, 1.04M pixels)
 against the oracle's accumulation and frame buffers, bit for bit.
@@ -37,7 +37,7 @@ def test_large_main_view_rows_match_oracle(device):
     import torch
     print("case: large shock scene (base 256^3, 5 levels), 3840x2160 main view, 1 spp", flush=True)
     t = time.time()
-    amr = AmrScene.shock(256, 4, 1200.0, 40.0, 0.74, 0.55, 1)
+    amr = AmrScene.shock(256, 4, 1200.0, 40.0, 0.74, 0.43, 1)
     ncells = len(amr)
     dm = DualMesh(amr)
     del amr
@@ -45,7 +45,7 @@ def test_large_main_view_rows_match_oracle(device):
     model = ExaStitchModel.from_dual_mesh(dm)
     del dm
     print(f"  scene: {ncells} cells, {len(idx)} elements, {len(gl)} gridlets ({time.time() - t:.0f}s)", flush=True)
-    assert ncells > 400_000_000 and len(idx) > 20_000_000
+    assert ncells > 370_000_000 and len(idx) > 0.14 * ncells
     t = time.time()
     r = OWLRenderer(model, (64, 64, 64))
     del model
