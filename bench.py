@@ -276,6 +276,21 @@ def batch_workload(r, args, dev):
     from owlexastitcher_amd import RENDER_PERSISTENT, RenderBatch, camera_look_at
     rng = np.random.default_rng(args.seed)
     vr = r.valueRange
+    # the single 1080p view's work per path sample and frame time, for the
+    # per-unit comparison below
+    stream = torch.cuda.current_stream()
+    fb1 = torch.zeros((args.height, args.width), dtype=torch.int32, device=dev)
+    for _ in range(3):
+        r.render(fb1, stream)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(10):
+        r.render(fb1, stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    single_frame_ms = e0.elapsed_time(e1) / 10
+    single_counts = r.render_counted()
+    del fb1
     tfs = [(random_tf(rng, int(rng.integers(5, 9))), float(rng.uniform(0.5, 4.0))) for _ in range(4)]
     b = r.modelBounds
     c = 0.5 * (b[:3] + b[3:])
@@ -302,10 +317,15 @@ def batch_workload(r, args, dev):
     best, best_ms = tune.autotuneLaunch((8, 16), (16, 32), (0, 8), (0, 1, 2), (2,), (0, 1 << 18), (128, 256))
     log(f"[bench] autotune on the 1-spp batch: best {best} ({best_ms:.1f} ms/frame), {time.time() - t:.1f}s")
     del tune
+    # the batch's work per path sample (one counted 1-spp frame, same tuning)
+    bc = make(1)
+    bc.setTuning(*best[:3])
+    bc.setLaunchTuning(*best[3:])
+    batch_counts = bc.render_counted()
+    del bc
     bt = make(args.batch_spp)
     bt.setTuning(*best[:3])
     bt.setLaunchTuning(*best[3:])
-    stream = torch.cuda.current_stream()
     for _ in range(args.warmup):
         bt.render(stream)
     torch.cuda.synchronize()
@@ -340,6 +360,18 @@ def batch_workload(r, args, dev):
         torch.cuda.synchronize()
         single_ms += e0.elapsed_time(e1)
     log(f"[bench] batch frame {np.mean(frame_ms):.1f} ms vs {single_ms:.1f} ms for the requests one at a time")
+
+    def per_sample(cnt, ms, spp):
+        n = cnt["pixels"] * spp
+        q = {k: cnt[k] / cnt["pixels"] for k in ("dda_cells", "tf", "gridlet", "cached", "pyr", "wedge", "hex")}
+        q["ns_per_path_sample"] = ms * 1e6 / n
+        q["ns_per_tf_fetch"] = ms * 1e6 / (cnt["tf"] * spp)
+        q["ns_per_dda_step"] = ms * 1e6 / (cnt["dda_cells"] * spp)
+        return q
+    work = {"batch": per_sample(batch_counts, float(np.mean(frame_ms)), args.batch_spp),
+            "single_1080p": per_sample(single_counts, single_frame_ms, 1)}
+    log(f"[bench] work per path sample: batch {work['batch']}")
+    log(f"[bench] work per path sample: 1080p view {work['single_1080p']}")
     return {
         "metric": METRIC, "value": samples * args.steps / wall, "unit": "samples/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3,
@@ -348,7 +380,7 @@ def batch_workload(r, args, dev):
                                f"cameras, 4 random 5-8 point TFs (opacity U[0.5,4]), {args.batch_spp} spp, one "
                                f"persistent launch per frame, same 3-level scene",
                    "pixels": pixels, "autotuned": list(best), "frame_ms": float(np.mean(frame_ms)),
-                   "single_view_sum_ms": single_ms},
+                   "single_view_sum_ms": single_ms, "work_per_path_sample": work},
         "roofline": None, "cpu_baseline": None,
     }
 
