@@ -11,8 +11,10 @@
 
 namespace exa {
 
-// budget of element-bin list entries (the bins are coarsened past it)
-static constexpr int64_t kMaxCandidates = 1ll << 28;
+// budgets of the element bins (coarsened past either): 2^29 bins (2 GB of
+// list offsets) and 2^30 list entries (12 GB)
+static constexpr int64_t kMaxBins = 1ll << 29;
+static constexpr int64_t kMaxCandidates = 1ll << 30;
 
 // bin index of coordinate x: the same float expression runs on the device
 static inline int32_t binOf(float x, float lo, float inv, int32_t dims) {
@@ -160,7 +162,7 @@ void buildLocateTables(const HostModel &m, LocateTables &t) {
   while (size < 1e-3f * std::max(std::max(hi[0] - lo[0], hi[1] - lo[1]), hi[2] - lo[2])) size *= 2.f;
   for (;;) {
     for (int k = 0; k < 3; ++k) t.binDims[k] = (int32_t)floorf((hi[k] - lo[k]) * (1.f / size)) + 1;
-    if ((int64_t)t.binDims[0] * t.binDims[1] * t.binDims[2] > (1ll << 27)) {
+    if ((int64_t)t.binDims[0] * t.binDims[1] * t.binDims[2] > kMaxBins) {
       size *= 2.f;
       continue;
     }
