@@ -8,6 +8,7 @@
 #define FMAX fmaxf
 #define SQRT sqrtf
 #define FABS fabsf
+#define FMA fmaf
 #define FLOOR floorf
 #define LOGF ora_logf
 #define SINCOS ora_sincosf

@@ -7,6 +7,7 @@
 #define FMAX fmax
 #define SQRT sqrt
 #define FABS fabs
+#define FMA fma
 #define FLOOR floor
 #define LOGF log
 #define SINCOS sincos

@@ -39,6 +39,8 @@ EXA_HD float length(vec3f a) { return sqrtf(dot(a, a)); }
 // owl normalize is v*rsqrt(dot(v,v)); taken here as a correctly rounded 1/sqrt.
 EXA_HD vec3f normalize(vec3f a) { return a * (1.f / sqrtf(dot(a, a))); }
 EXA_HD vec3f xyz(float4 v) { return mk3(v.x, v.y, v.z); }
+// c + a * s with one rounding per component (fused multiply-add)
+EXA_HD vec3f fma3(vec3f a, float s, vec3f c) { return mk3(fmaf(a.x, s, c.x), fmaf(a.y, s, c.y), fmaf(a.z, s, c.z)); }
 
 // float -> int32, round toward zero, saturating, NaN -> 0 (the conversion the
 // reference's index math relies on, e.g. Grid.cu:711-712, Grid.cuh:62).
@@ -366,7 +368,10 @@ struct ShapeFn<8> {
   }
 };
 
-// UElems.h:88-177 / 219-310 / 372-464
+// UElems.h:88-177 / 219-310 / 372-464.  The position and Jacobian sums and
+// the interpolated value accumulate by fused multiply-adds, one rounding per
+// term, as nvcc compiles them by default (--fmad=true); the oracle fuses the
+// same terms.
 template <int N>
 EXA_HD bool intersectNewton(float &value, vec3f P, const float4 *V) {
   const float determinantTolerance = 1e-6f;
@@ -381,10 +386,10 @@ EXA_HD bool intersectNewton(float &value, vec3f P, const float4 *V) {
 #pragma unroll
     for (int i = 0; i < N; ++i) {
       const vec3f pt = xyz(V[i]);
-      fcol = fcol + pt * weights[i];
-      rcol = rcol + pt * derivs[i];
-      scol = scol + pt * derivs[i + N];
-      tcol = tcol + pt * derivs[i + 2 * N];
+      fcol = fma3(pt, weights[i], fcol);
+      rcol = fma3(pt, derivs[i], rcol);
+      scol = fma3(pt, derivs[i + N], scol);
+      tcol = fma3(pt, derivs[i + 2 * N], tcol);
     }
     fcol = fcol - P;
     const float d = det3(rcol, scol, tcol);
@@ -414,7 +419,7 @@ EXA_HD bool intersectNewton(float &value, vec3f P, const float4 *V) {
   if (inside) {
     float val = 0.f;
 #pragma unroll
-    for (int i = 0; i < N; i++) val += weights[i] * V[i].w;
+    for (int i = 0; i < N; i++) val = fmaf(weights[i], V[i].w, val);
     value = val;
     return true;
   }
@@ -457,10 +462,10 @@ EXA_HD bool intersectNewtonAny(float &value, vec3f P, const float4 *V, int nv) {
     for (int i = 0; i < 8; ++i) {
       if (i < nv) {
         const vec3f pt = xyz(V[i]);
-        fcol = fcol + pt * w[i];
-        rcol = rcol + pt * dr[i];
-        scol = scol + pt * ds[i];
-        tcol = tcol + pt * dt[i];
+        fcol = fma3(pt, w[i], fcol);
+        rcol = fma3(pt, dr[i], rcol);
+        scol = fma3(pt, ds[i], scol);
+        tcol = fma3(pt, dt[i], tcol);
       }
     }
     fcol = fcol - P;
@@ -489,7 +494,7 @@ EXA_HD bool intersectNewtonAny(float &value, vec3f P, const float4 *V, int nv) {
     float val = 0.f;
 #pragma unroll
     for (int i = 0; i < 8; i++)
-      if (i < nv) val += w[i] * V[i].w;
+      if (i < nv) val = fmaf(w[i], V[i].w, val);
     value = val;
     return true;
   }
