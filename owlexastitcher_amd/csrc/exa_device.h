@@ -444,10 +444,25 @@ EXA_HD void shapeSlots(const float *pc, float w[8], float dr[8], float ds[8], fl
     dt[i] = i < N ? d[i + 2 * N] : 0.f;
   }
 }
+// Pyramids take the hexahedron's evaluation: in slots 0-3 the pyramid's
+// shape functions and derivatives are the hexahedron's bit for bit (with
+// fl(x - 1) = -fl(1 - x), e.g. -(s-1)*(t-1) = -(sm*tm) = -sm*tm), except
+// dr[2] = s - s*t and ds[2] = r - r*t; slot 4 is the apex (w = t, dt = 1).
+// So a wave holding pyramids and hexahedra evaluates one set of formulas.
 EXA_HD void shapeAny(int nv, const float *pc, float w[8], float dr[8], float ds[8], float dt[8]) {
-  if (nv == 8) shapeSlots<8>(pc, w, dr, ds, dt);
-  else if (nv == 6) shapeSlots<6>(pc, w, dr, ds, dt);
-  else shapeSlots<5>(pc, w, dr, ds, dt);
+  if (nv == 6) {
+    shapeSlots<6>(pc, w, dr, ds, dt);
+    return;
+  }
+  shapeSlots<8>(pc, w, dr, ds, dt);
+  if (nv == 5) {
+    dr[2] = pc[1] - pc[1] * pc[2];
+    ds[2] = pc[0] - pc[0] * pc[2];
+    w[4] = pc[2];
+    dr[4] = 0.f;
+    ds[4] = 0.f;
+    dt[4] = 1.f;
+  }
 }
 
 EXA_HD bool intersectNewtonAny(float &value, vec3f P, const float4 *V, int nv) {
