@@ -166,9 +166,10 @@ EXA_DEV Sample locateTablesGridlets(const SamplerView &sv, const Culling &cu, co
 // The Woodcock loop's sample lookup with the direct tables: testSample on
 // the cached gridlet (ExaStitchSampler.h:109-126) and, when that misses,
 // full point location (gridlet stage).  Same result as
-// testSampleGridlet() followed by locateTablesGridlets(), but the loads of
-// the cached-gridlet test and of every level's owner tables are issued
-// together, so a lookup costs about four dependent memory round trips.
+// testSampleGridlet() followed by locateTablesGridlets(), but the cached
+// gridlet's record and every level's owner tables are loaded together and a
+// lane runs one gridlet test (on the cached gridlet or on the owner), so a
+// lookup costs about four dependent memory round trips and one test.
 // cachedHit tells which of the two answered.
 constexpr int kFastLevels = 8;
 EXA_DEV Sample lookupTablesGridlets(const SamplerView &sv, const Culling &cu, const vec3f p, int32_t cached,
@@ -207,35 +208,53 @@ EXA_DEV Sample lookupTablesGridlets(const SamplerView &sv, const Culling &cu, co
     blk[L] = sv.gTop[idx];
     local[L] = (((int32_t)cz & 7) * 8 + ((int32_t)cy & 7)) * 8 + ((int32_t)cx & 7);
   }
-  // round 2: owners (the cached test's scalar loads go out with them)
+  // round 2: owners
   int32_t own[kFastLevels];
 #pragma unroll
   for (int L = 0; L < kFastLevels; ++L)
     own[L] = blk[L] >= 0 ? sv.gOwner[(int64_t)blk[L] * LocateTables::kOwnerBlock + local[L]] : -1;
-  if (cached >= 0) {
-    float v;
-    int cid;
-    if (intersectGridlet(v, cid, p, cg, sv.gridletScalars, sv.bandMargin)) {
+  // the lowest level whose owner answers, if no undecided level lies below
+  int32_t g = -1;
+  if (fast) {
+#pragma unroll
+    for (int L = kFastLevels - 1; L >= 0; --L)
+      if (own[L] >= 0 && L < undecided) g = own[L];
+  }
+  // The cached gridlet's test accepts only points inside the gridlet's box,
+  // the test's own first check.  So each lane runs one gridlet test: on the
+  // cached gridlet where p lies in its box, else on the owner (rounds 3 and
+  // 4: owner record and culling flag, then the corner scalars).  Only a
+  // cached gridlet that rejects a point inside its box (a hole there) leads
+  // to a second test, on the owner.
+  vec3f clo, chi;
+  gridletBounds(cg, clo, chi);
+  const bool inCached = cached >= 0 && p.x >= clo.x && p.y >= clo.y && p.z >= clo.z && p.x <= chi.x &&
+                        p.y <= chi.y && p.z <= chi.z;
+  const bool haveOwner = fast && g >= 0;
+  bool ownerTest = false;
+  if (!inCached && haveOwner) ownerTest = !(cu.gridletMaxOp && cu.gridletMaxOp[g] == 0.f);
+  float v = 0.f;
+  int cid = -1;
+  int r = 0;
+  if (inCached || ownerTest) {
+    Gridlet gt = cg;
+    if (!inCached) gt = sv.gridlets[g];
+    r = intersectGridlet(v, cid, p, gt, sv.gridletScalars, sv.bandMargin);
+  }
+  if (inCached) {
+    if (r) {
       cachedHit = true;
       return Sample{cached, cid, v};
     }
+    ownerTest = haveOwner && !(cu.gridletMaxOp && cu.gridletMaxOp[g] == 0.f);
+    r = ownerTest ? intersectGridlet(v, cid, p, sv.gridlets[g], sv.gridletScalars, sv.bandMargin) : 0;
   }
   if (!fast) return locateTablesGridlets(sv, cu, p);
-  // the lowest level whose owner answers, if no undecided level lies below
-  int32_t g = -1;
-#pragma unroll
-  for (int L = kFastLevels - 1; L >= 0; --L)
-    if (own[L] >= 0 && L < undecided) g = own[L];
   if (g < 0) {
     if (undecided == kFastLevels) return Sample{-1, -1, 0.f};
     return locateTablesGridlets(sv, cu, p);
   }
-  // rounds 3 and 4: owner record and culling flag, then its corner scalars
-  if (!(cu.gridletMaxOp && cu.gridletMaxOp[g] == 0.f)) {
-    float v;
-    int cid;
-    if (intersectGridlet(v, cid, p, sv.gridlets[g], sv.gridletScalars, sv.bandMargin) == 2) return Sample{g, cid, v};
-  }
+  if (ownerTest && r == 2) return Sample{g, cid, v};
   return locateTablesGridlets(sv, cu, p);
 }
 
