@@ -41,9 +41,11 @@ float ora_lcg_next(uint32_t *state) {
 /* Own single-precision log and sin/cos (the same algorithms run in the    */
 /* HIP kernels, so the oracle and the device agree bit for bit).            */
 /*  logf: Cody & Waite argument reduction x = m*2^e, m in [sqrt(1/2),       */
-/*        sqrt(2)); log(m) = 2*atanh(s), s = (m-1)/(m+1), odd series to     */
-/*        s^9 (Cody & Waite, "Software Manual for the Elementary Functions",*/
-/*        1980).  Domain: positive normal floats.                           */
+/*        sqrt(2)) (Cody & Waite, "Software Manual for the Elementary       */
+/*        Functions", 1980); log(m) = f + f^2 Q(f), f = m - 1, Q of degree  */
+/*        8 fitted to (log1p(f) - f)/f^2 by reweighted least squares        */
+/*        (fit_logf.py), Horner by fused multiply-adds, no division.        */
+/*        Domain: positive normal floats.                                   */
 /*  sincosf: reduction by the nearest multiple of pi/2 in double, Taylor    */
 /*        polynomials on [-pi/4, pi/4] (sin to r^9, cos to r^10).  Domain:  */
 /*        |x| < 1e6.                                                        */
@@ -59,14 +61,18 @@ float ora_logf(float x) {
     e += 1;
   }
   const float f = m - 1.0f;
-  const float s = f / (2.0f + f);
-  const float z = s * s;
-  const float Rz = z * (0.666666686534881591796875f +
-                        z * (0.4000000059604644775390625f +
-                             z * (0.2857142984867095947265625f + z * 0.2222222238779067993164062f)));
-  const float logm = 2.0f * s + s * Rz;
+  float q = -0.07823377847671509f;
+  q = fmaf(q, f, 0.13004831969738007f);
+  q = fmaf(q, f, -0.13208796083927155f);
+  q = fmaf(q, f, 0.14163468778133392f);
+  q = fmaf(q, f, -0.16611149907112122f);
+  q = fmaf(q, f, 0.2000301629304886f);
+  q = fmaf(q, f, -0.2500152289867401f);
+  q = fmaf(q, f, 0.3333330750465393f);
+  q = fmaf(q, f, -0.49999988079071045f);
+  const float logm = fmaf(f * f, q, f);
   const float fe = (float)e;
-  return fe * 0.693145751953125f + (fe * 1.428606765330187045e-06f + logm);
+  return fmaf(fe, 0.693145751953125f, fmaf(fe, 1.428606765330187045e-06f, logm));
 }
 
 void ora_sincosf(float x, float *sn, float *cs) {

@@ -35,7 +35,8 @@ def test_logf_error_bound():
     rng = np.random.default_rng(1)
     # the device only evaluates log(1 - xi), xi = k / 2^24
     xs = np.concatenate([1.0 - np.arange(0, 1 << 24, 997, dtype=np.float64) / 2 ** 24,
-                         rng.uniform(1e-6, 1.0, 20000), [1.0, 0.5, 2.0 ** -24, 0.7071067811865476]])
+                         rng.uniform(1e-6, 1.0, 20000), rng.uniform(1e-30, 1.0, 5000), rng.uniform(1.0, 1e30, 5000),
+                         [1.0, 0.5, 2.0 ** -24, 0.7071067811865476, 1.4142135]])
     worst = 0.0
     for x in xs.astype(np.float32):
         got = L().ora_logf(float(x))
@@ -44,7 +45,8 @@ def test_logf_error_bound():
             assert got == 0.0
             continue
         worst = max(worst, ulp_err(got, ref))
-    assert worst <= 2.0, worst
+    # the fitted polynomial measures 0.82 ulp at worst over these points
+    assert worst <= 1.0, worst
 
 
 def test_sincosf_error_bound():
