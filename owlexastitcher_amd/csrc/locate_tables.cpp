@@ -11,9 +11,12 @@
 
 namespace exa {
 
-// budgets of the element bins (coarsened past either): 2^29 bins (2 GB of
-// list offsets) and 2^30 list entries (12 GB)
+// budgets of the element bins (coarsened past any of them): 2^29 bins (2 GB
+// of list offsets), at most 32 bins per element (at least 2^20), so that a
+// few small elements in a large domain do not allocate mostly empty bins,
+// and 2^30 list entries (12 GB)
 static constexpr int64_t kMaxBins = 1ll << 29;
+static constexpr int64_t kBinsPerElement = 32;
 static constexpr int64_t kMaxCandidates = 1ll << 30;
 
 // bin index of coordinate x: the same float expression runs on the device
@@ -162,7 +165,8 @@ void buildLocateTables(const HostModel &m, LocateTables &t) {
   while (size < 1e-3f * std::max(std::max(hi[0] - lo[0], hi[1] - lo[1]), hi[2] - lo[2])) size *= 2.f;
   for (;;) {
     for (int k = 0; k < 3; ++k) t.binDims[k] = (int32_t)floorf((hi[k] - lo[k]) * (1.f / size)) + 1;
-    if ((int64_t)t.binDims[0] * t.binDims[1] * t.binDims[2] > kMaxBins) {
+    if ((int64_t)t.binDims[0] * t.binDims[1] * t.binDims[2] >
+        std::min<int64_t>(kMaxBins, std::max<int64_t>(1ll << 20, kBinsPerElement * ne))) {
       size *= 2.f;
       continue;
     }
