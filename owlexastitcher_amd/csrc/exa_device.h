@@ -266,7 +266,12 @@ EXA_HD bool intersectTet(float &value, vec3f P, float4 a, float4 b, float4 c, fl
 // UElems.h:40-44 det; pyramid 46-177, wedge 180-310, hex 313-464 (Newton on
 // the isoparametric map, adapted there from OpenVKL)
 // ---------------------------------------------------------------------------
-EXA_HD float det3(vec3f x, vec3f y, vec3f z) { return dot(x, cross(y, z)); }
+// x . (y x z) of the Newton solve, each cross-product component and the dot
+// product's sums by fused multiply-adds (the oracle fuses the same terms)
+EXA_HD float det3(vec3f x, vec3f y, vec3f z) {
+  const vec3f c = mk3(fmaf(y.y, z.z, -(y.z * z.y)), fmaf(y.z, z.x, -(y.x * z.z)), fmaf(y.x, z.y, -(y.y * z.x)));
+  return fmaf(x.z, c.z, fmaf(x.y, c.y, x.x * c.x));
+}
 
 template <int N>
 struct ShapeFn;
