@@ -330,3 +330,27 @@ def test_locate_binned_equals_brute_force():
     b, cnt = om.locate_brute(pos)
     assert np.array_equal(b, om.locate_binned(pos))
     assert (cnt >= 1).sum() > 2500  # hull [1,31]^3 of [-1,33]^3: 69% of the points
+
+
+def test_logf_coefficients_come_from_the_fit():
+    """The polynomial in ora_logf (and exa_logf) is the one oracle/fit_logf.py
+    derives; the device copy holds the same constants."""
+    import os
+    import re
+    from oracle import fit_logf
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+    def constants(path, fn):
+        src = open(os.path.join(root, path)).read()
+        body = src[src.index(fn):]
+        body = body[:body.index("\n}\n")]
+        first = re.search(r"float q = (-?[0-9.e+-]+)f;", body).group(1)
+        rest = re.findall(r"q = fmaf\(q, f, (-?[0-9.e+-]+)f\);", body)
+        return [float(x) for x in [first] + rest][::-1]  # lowest degree first
+
+    fitted = fit_logf.fit().astype(np.float64)
+    for path, fn in (("oracle/ora_f32.c", "float ora_logf("),
+                     ("owlexastitcher_amd/csrc/exa_device.h", "EXA_HD float exa_logf(")):
+        c = np.array(constants(path, fn))
+        assert len(c) == len(fitted) == 9
+        assert np.allclose(c, fitted, rtol=1e-6, atol=0), (path, c, fitted)
