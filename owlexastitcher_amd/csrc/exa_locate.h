@@ -266,14 +266,11 @@ EXA_DEV Sample lookupTablesGridlets(const SamplerView &sv, const Culling &cu, co
 // (tet/pyr/wedge/hex).
 EXA_DEV int testElementBoxed(float &value, vec3f P, int nv, const float4 *__restrict__ rec, bool boxCheck) {
   if (nv < 4) return -1;
+  // all 8 slots, whatever nv (the record is 128 bytes with zeros past nv):
+  // one run of independent loads instead of loads behind per-lane branches
   float4 v[8];
-  v[0] = rec[0];
-  v[1] = rec[1];
-  v[2] = rec[2];
-  v[3] = rec[3];
-  if (nv > 4) v[4] = rec[4];
-  if (nv > 5) v[5] = rec[5];
-  if (nv > 6) { v[6] = rec[6]; v[7] = rec[7]; }
+#pragma unroll
+  for (int i = 0; i < 8; ++i) v[i] = rec[i];
   if (boxCheck) {
     float lx = v[0].x, ly = v[0].y, lz = v[0].z, hx = lx, hy = ly, hz = lz;
 #pragma unroll
